@@ -1,0 +1,140 @@
+/*
+ * pxd -- C ABI of the MI355X expander-decomposition engine (HIP, gfx950).
+ *
+ * Drop-in boundary for the reference's CUDA expander decomposition: every
+ * entry point below replaces one member of `class CudaDeviceManager`
+ * (cuda/interface.h:16-57, forwarders cuda/device.h:394-427 of the reference).
+ * The reference reads its parameters from mutable globals (randSeed,
+ * sc_threshold, rw_threshold; lib/core/definitions.h:46-50); here they are
+ * passed explicitly in pxd_params.
+ *
+ * Conventions: plain pointers and sizes only; every call returns 0 on success
+ * or a negative pxd_status; pxd_last_error() gives the message of the last
+ * failure on the calling thread.  A handle is not thread-safe; use one per
+ * host thread.  All device work of a handle runs on the handle's own stream.
+ * Output buffers are caller-owned host memory.
+ */
+#ifndef PAREX_AMD_PXD_H
+#define PAREX_AMD_PXD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXD_INVALID_EDGE 0xFFFFFFFFu   /* lib/core/definitions.h:52 */
+
+typedef enum {
+    PXD_OK = 0,
+    PXD_ERR_ARG = -1,        /* bad argument / call order */
+    PXD_ERR_HIP = -2,        /* HIP runtime error */
+    PXD_ERR_OOM = -3,        /* device allocation failed */
+    PXD_ERR_NUMERIC = -4,    /* walk value outside the exact fixed-point range */
+    PXD_ERR_STATE = -5       /* internal invariant violated */
+} pxd_status;
+
+/* lib/core/definitions.h:57-84 -- identical byte layouts */
+typedef struct { int32_t clusterId; float sparsity; uint32_t offset; } pxd_sweep_cut;       /* SweepCutData */
+typedef struct { uint32_t nix; int32_t clusterId; } pxd_labeled_node;                       /* LabeledNode */
+typedef struct { uint32_t nix; uint32_t prefixVolume; int32_t prefixEdgeDiff; uint32_t offsetInCluster; } pxd_node_data; /* NodeData */
+
+typedef struct {
+    uint32_t seed;           /* randSeed, default 6582 */
+    float sc_threshold;      /* sweep-cut (conductance) threshold, default 0.3 */
+    float rw_threshold;      /* random-walk potential threshold, default 1e-4 */
+    uint32_t max_iter;       /* MAX_NUM_ITER, default 200000 (cuda/device.h:309) */
+    uint32_t keep_original;  /* keep a copy of the input CSR for pxd_download_graph /
+                                pxd_download_active_edge_map (default 1) */
+    uint32_t record_node_data; /* keep per-position NodeData for pxd_download_partition (default 1) */
+    int32_t device;          /* HIP device ordinal, default 0 */
+} pxd_params;
+
+typedef struct pxd_engine pxd_engine;
+
+void pxd_default_params(pxd_params* p);
+const char* pxd_last_error(void);
+
+/* CudaDeviceManager() / ~CudaDeviceManager()   (cuda/interface.h:18-19) */
+int pxd_create(const pxd_params* params, pxd_engine** out);
+int pxd_destroy(pxd_engine* e);
+
+/* initialize(const Graph&)  (cuda/interface.h:24, cuda/device.h:24-38):
+ * copies the CSR (ranges u32[n+1], edges u32[2m], rows sorted, both
+ * directions present) to the device and builds all state, including the
+ * initial walk vector. */
+int pxd_initialize(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges);
+
+/* Batched variant (no reference counterpart: the reference handles one graph
+ * per CudaDeviceManager).  num_graphs independent graphs, each given in its
+ * own local numbering; graph g has node_counts[g] nodes, edge_counts[g]
+ * undirected edges, ranges_ptrs[g] (n_g+1 entries) and edges_ptrs[g] (2 m_g).
+ * Every graph's result equals what pxd_initialize + run gives for it alone. */
+int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* node_counts,
+                         const uint32_t* edge_counts, const uint32_t* const* ranges_ptrs,
+                         const uint32_t* const* edges_ptrs);
+
+/* iterateRandomWalk()   (cuda/interface.h:29, cuda/device.h:40-48) */
+int pxd_iterate_random_walk(pxd_engine* e);
+/* computeSweepCuts()    (cuda/interface.h:32, cuda/device.h:77-81) */
+int pxd_compute_sweep_cuts(pxd_engine* e);
+/* cutClusters()         (cuda/interface.h:36, cuda/device.h:101-168) */
+int pxd_cut_clusters(pxd_engine* e);
+/* expanderDecomposition() (cuda/interface.h:40, cuda/device.h:301-350):
+ * runs the loop to completion; *iterations receives the value the reference
+ * prints ("iterations: %d"); with a batch, the largest per-graph value. */
+int pxd_expander_decomposition(pxd_engine* e, int32_t* iterations);
+/* per-graph "iterations" values after pxd_expander_decomposition */
+int pxd_graph_iterations(pxd_engine* e, int32_t* out, uint32_t num_graphs);
+
+/* getFinalPartition()   (cuda/interface.h:42, cuda/devPartition.h:798-823):
+ * ids_out gets n dense cluster ids; *num_clusters the number of ids.
+ * For a batch, graph g's slice holds ids local to graph g and
+ * num_clusters[g] is per graph (pass num_graphs entries). */
+int pxd_get_final_partition(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters);
+
+/* readRandomWalkValues() (cuda/interface.h:30): n floats (FP64 state cast) */
+int pxd_read_random_walk_values(pxd_engine* e, float* out);
+/* the FP64 state itself (no reference counterpart; used for parity tests) */
+int pxd_read_random_walk_values_f64(pxd_engine* e, double* out);
+/* downloadDegrees() (cuda/interface.h:26): internal degrees, n entries */
+int pxd_download_degrees(pxd_engine* e, uint32_t* out);
+/* downloadLabels() (cuda/interface.h:45): allLabels, n entries */
+int pxd_download_labels(pxd_engine* e, int32_t* out);
+/* downloadGraph() (cuda/interface.h:25): ranges n+1, edges 2m with disabled
+ * arcs written as PXD_INVALID_EDGE in their original positions */
+int pxd_download_graph(pxd_engine* e, uint32_t* ranges_out, uint32_t* edges_out);
+/* downloadActiveEdgeMap() (cuda/interface.h:44): 2m ints, 1 = arc still valid */
+int pxd_download_active_edge_map(pxd_engine* e, int32_t* out);
+/* downloadPartition() (cuda/interface.h:27): NodeData of the last sweep for the
+ * first *count active positions (capacity entries available in out) */
+int pxd_download_partition(pxd_engine* e, pxd_node_data* out, uint32_t capacity, uint32_t* count);
+/* readSweepCuts() (cuda/interface.h:34, cuda/device.h:170-179): labels and
+ * best cuts of the active clusters of the last sweep */
+int pxd_read_sweep_cuts(pxd_engine* e, int32_t* cluster_ids, pxd_sweep_cut* cuts,
+                        uint32_t capacity, uint32_t* count);
+/* getNumCutEdges() (cuda/interface.h:38, cuda/devPartition.h:741-759): m - validArcs/2 */
+int pxd_get_num_cut_edges(pxd_engine* e, uint32_t* out);
+/* active list (activeNodes) in device order; capacity entries */
+int pxd_download_active_nodes(pxd_engine* e, pxd_labeled_node* out, uint32_t capacity, uint32_t* count);
+/* counters: numActiveNodes, numActiveClusters, numClustersWithCut */
+int pxd_get_counts(pxd_engine* e, uint32_t* num_active_nodes, uint32_t* num_active_clusters,
+                   uint32_t* num_clusters_with_cut);
+/* overwrite the FP64 walk state of all n nodes (the reference's tests seed the
+ * CPU walk from the device; this is the converse, for parity runs) */
+int pxd_set_random_walk_values_f64(pxd_engine* e, const double* values);
+
+/* device time spent inside the last pxd_expander_decomposition (ms, HIP events
+ * on the handle's stream) and per-kernel-class accumulated times */
+int pxd_last_run_ms(pxd_engine* e, double* ms);
+
+/* Per-kernel timing of the walk kernel: enable HIP-event timing around every
+ * walk launch; pxd_walk_stats returns launches, total ms and the algorithmic
+ * bytes (4+8 per valid arc + 32 per active node) summed over those launches. */
+int pxd_enable_walk_timing(pxd_engine* e, int enable);
+int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,445 @@
+"""parex_amd -- MI355X-native expander decomposition (HIP, gfx950).
+
+Python host mirror of the reference's GPU interface (``class CudaDeviceManager``,
+cuda/interface.h:16-57 of RobinMnk/parex) over the engine's C ABI
+(``include/parex_amd/pxd.h``), plus the host graph ingest library
+(``include/parex_amd/pxgen.h``).  Both are loaded from shared libraries built
+in-tree by ``parex_amd.build``; nothing here falls back to a CPU path: a
+missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+INVALID_EDGE = 0xFFFFFFFF
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+# lib/core/definitions.h:57-77 layouts
+SWEEP_CUT_DTYPE = np.dtype([("clusterId", np.int32), ("sparsity", np.float32), ("offset", np.uint32)])
+NODE_DATA_DTYPE = np.dtype([("nix", np.uint32), ("prefixVolume", np.uint32),
+                            ("prefixEdgeDiff", np.int32), ("offsetInCluster", np.uint32)])
+LABELED_NODE_DTYPE = np.dtype([("nix", np.uint32), ("clusterId", np.int32)])
+
+
+class PxdError(RuntimeError):
+    """Error reported by the engine's C ABI (status code + message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pxd error {code}: {msg}")
+        self.code = code
+
+
+# --------------------------------------------------------------------------- libraries
+
+class _PxgGraph(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint32),
+                ("ranges", C.POINTER(C.c_uint32)), ("edges", C.POINTER(C.c_uint32))]
+
+
+class _SbmSpec(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("k", C.c_uint32), ("p_in", C.c_double),
+                ("p_out", C.c_double), ("seed", C.c_uint64)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("sc_threshold", C.c_float), ("rw_threshold", C.c_float),
+                ("max_iter", C.c_uint32), ("keep_original", C.c_uint32),
+                ("record_node_data", C.c_uint32), ("device", C.c_int32)]
+
+
+_libs = {}
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(PKG_DIR, name)
+
+
+def _load(name: str) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = lib_path(name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "or `python -m parex_amd.build`")
+    lib = C.CDLL(path)
+    _libs[name] = lib
+    return lib
+
+
+def pxgen() -> C.CDLL:
+    lib = _load("libpxgen.so")
+    if not getattr(lib, "_typed", False):
+        G = C.POINTER(_PxgGraph)
+        lib.pxg_sbm.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64, G]
+        lib.pxg_torus.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, G]
+        lib.pxg_rmat.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64, G]
+        lib.pxg_sbm_specs.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(_SbmSpec)]
+        lib.pxg_sbm_batch.argtypes = [C.c_uint32, C.POINTER(_SbmSpec), G]
+        lib.pxg_read_mtx.argtypes = [C.c_char_p, G]
+        lib.pxg_write.argtypes = [C.c_char_p, G]
+        lib.pxg_read.argtypes = [C.c_char_p, G]
+        lib.pxg_from_edges.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, G]
+        lib.pxg_verify_partition.argtypes = [G, _i32p, C.c_int32, C.POINTER(C.c_uint64), C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
+        lib.pxg_free.argtypes = [G]
+        lib.pxg_last_error.restype = C.c_char_p
+        lib._typed = True
+    return lib
+
+
+def pxd() -> C.CDLL:
+    lib = _load("libpxd.so")
+    if not getattr(lib, "_typed", False):
+        P = C.c_void_p
+        lib.pxd_default_params.argtypes = [C.POINTER(_Params)]
+        lib.pxd_last_error.restype = C.c_char_p
+        lib.pxd_create.argtypes = [C.POINTER(_Params), C.POINTER(C.c_void_p)]
+        lib.pxd_destroy.argtypes = [P]
+        lib.pxd_initialize.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p]
+        lib.pxd_initialize_batch.argtypes = [P, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_void_p),
+                                             C.POINTER(C.c_void_p)]
+        for f in ("pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters"):
+            getattr(lib, f).argtypes = [P]
+        lib.pxd_expander_decomposition.argtypes = [P, C.POINTER(C.c_int32)]
+        lib.pxd_graph_iterations.argtypes = [P, _i32p, C.c_uint32]
+        lib.pxd_get_final_partition.argtypes = [P, _i32p, _i32p]
+        lib.pxd_read_random_walk_values.argtypes = [P, _f32p]
+        lib.pxd_read_random_walk_values_f64.argtypes = [P, _f64p]
+        lib.pxd_set_random_walk_values_f64.argtypes = [P, _f64p]
+        lib.pxd_download_degrees.argtypes = [P, _u32p]
+        lib.pxd_download_labels.argtypes = [P, _i32p]
+        lib.pxd_download_graph.argtypes = [P, _u32p, _u32p]
+        lib.pxd_download_active_edge_map.argtypes = [P, _i32p]
+        lib.pxd_download_partition.argtypes = [P, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
+        lib.pxd_read_sweep_cuts.argtypes = [P, _i32p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
+        lib.pxd_get_num_cut_edges.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.pxd_download_active_nodes.argtypes = [P, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
+        lib.pxd_get_counts.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.pxd_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
+        lib.pxd_enable_walk_timing.argtypes = [P, C.c_int]
+        lib.pxd_walk_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib._typed = True
+    return lib
+
+
+# C-ABI entry points declared in include/parex_amd/pxd.h (checked by the tests)
+PXD_SYMBOLS = [
+    "pxd_default_params", "pxd_last_error", "pxd_create", "pxd_destroy", "pxd_initialize",
+    "pxd_initialize_batch", "pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters",
+    "pxd_expander_decomposition", "pxd_graph_iterations", "pxd_get_final_partition",
+    "pxd_read_random_walk_values", "pxd_read_random_walk_values_f64", "pxd_download_degrees",
+    "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
+    "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
+    "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
+]
+
+
+# --------------------------------------------------------------------------- graphs
+
+@dataclass
+class Graph:
+    """CSR graph (lib/core/graph.h:145-158): ranges u32[n+1], edges u32[2m]."""
+
+    n: int
+    m: int
+    ranges: np.ndarray
+    edges: np.ndarray
+
+    def degree(self) -> np.ndarray:
+        return np.diff(self.ranges).astype(np.uint32)
+
+
+def _take(g: _PxgGraph) -> Graph:
+    lib = pxgen()
+    n, m = int(g.n), int(g.m)
+    ranges = np.ctypeslib.as_array(g.ranges, shape=(n + 1,)).copy() if g.ranges else np.zeros(1, np.uint32)
+    edges = (np.ctypeslib.as_array(g.edges, shape=(2 * m,)).copy() if (g.edges and m)
+             else np.zeros(0, np.uint32))
+    lib.pxg_free(C.byref(g))
+    return Graph(n, m, ranges.astype(np.uint32), edges.astype(np.uint32))
+
+
+def _gen_check(rc: int):
+    if rc != 0:
+        raise RuntimeError(pxgen().pxg_last_error().decode())
+
+
+def sbm(n: int, k: int, p_in: float, p_out: float, seed: int) -> Graph:
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_sbm(n, k, p_in, p_out, seed, C.byref(g)))
+    return _take(g)
+
+
+def torus(L: int, extra: int, seed: int) -> Graph:
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_torus(L, extra, seed, C.byref(g)))
+    return _take(g)
+
+
+def rmat(scale: int, ef: int, a: float, b: float, c: float, seed: int) -> Graph:
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_rmat(scale, ef, a, b, c, seed, C.byref(g)))
+    return _take(g)
+
+
+def sbm_specs(count, nmin, nmax, kmin, kmax, din, dout, seed):
+    arr = (_SbmSpec * count)()
+    _gen_check(pxgen().pxg_sbm_specs(count, nmin, nmax, kmin, kmax, din[0], din[1], dout[0], dout[1], seed, arr))
+    return arr
+
+
+def sbm_batch(specs) -> List[Graph]:
+    count = len(specs)
+    out = (_PxgGraph * count)()
+    _gen_check(pxgen().pxg_sbm_batch(count, specs, out))
+    return [_take(out[i]) for i in range(count)]
+
+
+def from_edges(n: int, fx: Sequence[int], tx: Sequence[int]) -> Graph:
+    fx = np.ascontiguousarray(fx, dtype=np.uint32)
+    tx = np.ascontiguousarray(tx, dtype=np.uint32)
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_from_edges(n, len(fx), fx, tx, C.byref(g)))
+    return _take(g)
+
+
+def read_mtx(path: str) -> Graph:
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_read_mtx(path.encode(), C.byref(g)))
+    return _take(g)
+
+
+def write_pxg(path: str, graph: Graph):
+    g = _PxgGraph(graph.n, graph.m, graph.ranges.ctypes.data_as(C.POINTER(C.c_uint32)),
+                  graph.edges.ctypes.data_as(C.POINTER(C.c_uint32)))
+    _gen_check(pxgen().pxg_write(path.encode(), C.byref(g)))
+
+
+def read_pxg(path: str) -> Graph:
+    g = _PxgGraph()
+    _gen_check(pxgen().pxg_read(path.encode(), C.byref(g)))
+    return _take(g)
+
+
+def verify_partition(graph: Graph, ids: np.ndarray, num_clusters: int):
+    """Host re-verification (main.cpp:53-60): returns (cut_edges, vol, internal_vol, conductance)."""
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    g = _PxgGraph(graph.n, graph.m, graph.ranges.ctypes.data_as(C.POINTER(C.c_uint32)),
+                  graph.edges.ctypes.data_as(C.POINTER(C.c_uint32)))
+    cut = C.c_uint64()
+    vol = np.zeros(max(num_clusters, 1), np.uint64)
+    ivol = np.zeros(max(num_clusters, 1), np.uint64)
+    cond = np.zeros(max(num_clusters, 1), np.float64)
+    _gen_check(pxgen().pxg_verify_partition(C.byref(g), ids, num_clusters, C.byref(cut), vol.ctypes.data,
+                                            ivol.ctypes.data, cond.ctypes.data))
+    return int(cut.value), vol[:num_clusters], ivol[:num_clusters], cond[:num_clusters]
+
+
+# --------------------------------------------------------------------------- engine
+
+@dataclass
+class Params:
+    seed: int = 6582                 # lib/core/definitions.h:46
+    sc_threshold: float = 0.3        # lib/core/definitions.h:50
+    rw_threshold: float = 1e-4       # lib/core/definitions.h:49
+    max_iter: int = 200000           # cuda/device.h:309
+    keep_original: bool = True
+    record_node_data: bool = True
+    device: int = 0
+
+
+class DeviceManager:
+    """Mirror of ``CudaDeviceManager`` (cuda/interface.h:16-57) over the HIP engine.
+
+    Method names follow the reference (camelCase) with snake_case aliases.
+    Errors raise :class:`PxdError`; the reference reported none.
+    """
+
+    def __init__(self, params: Params | None = None):
+        self._lib = pxd()
+        self.params = params or Params()
+        p = _Params(self.params.seed, self.params.sc_threshold, self.params.rw_threshold, self.params.max_iter,
+                    int(self.params.keep_original), int(self.params.record_node_data), self.params.device)
+        h = C.c_void_p()
+        self._check(self._lib.pxd_create(C.byref(p), C.byref(h)))
+        self._h = h
+        self._n = 0
+        self._m = 0
+        self._graphs = 1
+        self._node_counts = None
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise PxdError(rc, self._lib.pxd_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pxd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # cuda/interface.h:24
+    def initialize(self, graph: Graph):
+        self._n, self._m, self._graphs = graph.n, graph.m, 1
+        self._node_counts = np.array([graph.n], np.uint32)
+        self._check(self._lib.pxd_initialize(self._h, graph.n, graph.m,
+                                             np.ascontiguousarray(graph.ranges, np.uint32),
+                                             np.ascontiguousarray(graph.edges, np.uint32)))
+
+    def initialize_batch(self, graphs: Sequence[Graph]):
+        G = len(graphs)
+        nc = np.array([g.n for g in graphs], np.uint32)
+        ec = np.array([g.m for g in graphs], np.uint32)
+        rs = [np.ascontiguousarray(g.ranges, np.uint32) for g in graphs]
+        es = [np.ascontiguousarray(g.edges, np.uint32) if g.m else np.zeros(1, np.uint32) for g in graphs]
+        rp = (C.c_void_p * G)(*[r.ctypes.data for r in rs])
+        ep = (C.c_void_p * G)(*[e.ctypes.data for e in es])
+        self._n, self._m, self._graphs = int(nc.sum()), int(ec.sum()), G
+        self._node_counts = nc
+        self._check(self._lib.pxd_initialize_batch(self._h, G, nc, ec, rp, ep))
+
+    def iterateRandomWalk(self):
+        self._check(self._lib.pxd_iterate_random_walk(self._h))
+
+    def computeSweepCuts(self):
+        self._check(self._lib.pxd_compute_sweep_cuts(self._h))
+
+    def cutClusters(self):
+        self._check(self._lib.pxd_cut_clusters(self._h))
+
+    def expanderDecomposition(self) -> int:
+        it = C.c_int32()
+        self._check(self._lib.pxd_expander_decomposition(self._h, C.byref(it)))
+        return int(it.value)
+
+    def graphIterations(self) -> np.ndarray:
+        out = np.zeros(self._graphs, np.int32)
+        self._check(self._lib.pxd_graph_iterations(self._h, out, self._graphs))
+        return out
+
+    def getFinalPartition(self):
+        ids = np.zeros(max(self._n, 1), np.int32)
+        nc = np.zeros(self._graphs, np.int32)
+        self._check(self._lib.pxd_get_final_partition(self._h, ids, nc))
+        ids = ids[: self._n]
+        if self._graphs == 1:
+            return ids, int(nc[0])
+        return ids, nc
+
+    def readRandomWalkValues(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), np.float32)
+        self._check(self._lib.pxd_read_random_walk_values(self._h, out))
+        return out[: self._n]
+
+    def readRandomWalkValuesF64(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), np.float64)
+        self._check(self._lib.pxd_read_random_walk_values_f64(self._h, out))
+        return out[: self._n]
+
+    def setRandomWalkValuesF64(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, np.float64)
+        self._check(self._lib.pxd_set_random_walk_values_f64(self._h, v))
+
+    def downloadDegrees(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), np.uint32)
+        self._check(self._lib.pxd_download_degrees(self._h, out))
+        return out[: self._n]
+
+    def downloadLabels(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), np.int32)
+        self._check(self._lib.pxd_download_labels(self._h, out))
+        return out[: self._n]
+
+    def downloadGraph(self) -> Graph:
+        r = np.zeros(self._n + self._graphs, np.uint32)
+        e = np.zeros(max(2 * self._m, 1), np.uint32)
+        self._check(self._lib.pxd_download_graph(self._h, r, e))
+        return Graph(self._n, self._m, r[: self._n + 1], e[: 2 * self._m])
+
+    def downloadActiveEdgeMap(self) -> np.ndarray:
+        out = np.zeros(max(2 * self._m, 1), np.int32)
+        self._check(self._lib.pxd_download_active_edge_map(self._h, out))
+        return out[: 2 * self._m]
+
+    def downloadPartition(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), NODE_DATA_DTYPE)
+        cnt = C.c_uint32()
+        self._check(self._lib.pxd_download_partition(self._h, out.ctypes.data, max(self._n, 1), C.byref(cnt)))
+        return out[: cnt.value]
+
+    def readSweepCuts(self) -> Tuple[np.ndarray, np.ndarray]:
+        ids = np.zeros(max(self._n, 1), np.int32)
+        cuts = np.zeros(max(self._n, 1), SWEEP_CUT_DTYPE)
+        cnt = C.c_uint32()
+        self._check(self._lib.pxd_read_sweep_cuts(self._h, ids, cuts.ctypes.data, max(self._n, 1), C.byref(cnt)))
+        return ids[: cnt.value], cuts[: cnt.value]
+
+    def getNumCutEdges(self) -> int:
+        out = C.c_uint32()
+        self._check(self._lib.pxd_get_num_cut_edges(self._h, C.byref(out)))
+        return int(out.value)
+
+    def downloadActiveNodes(self) -> np.ndarray:
+        out = np.zeros(max(self._n, 1), LABELED_NODE_DTYPE)
+        cnt = C.c_uint32()
+        self._check(self._lib.pxd_download_active_nodes(self._h, out.ctypes.data, max(self._n, 1), C.byref(cnt)))
+        return out[: cnt.value]
+
+    def counts(self) -> Tuple[int, int, int]:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.pxd_get_counts(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def getNumClusters(self) -> int:
+        """Returns 0 like the reference stub (cuda/device.h:58-61)."""
+        return 0
+
+    def updateLabels(self, node_labels, active_clusters):
+        """No-op like the reference (cuda/devGraph.h:191-206)."""
+        return None
+
+    def lastRunMs(self) -> float:
+        v = C.c_double()
+        self._check(self._lib.pxd_last_run_ms(self._h, C.byref(v)))
+        return float(v.value)
+
+    def enableWalkTiming(self, on: bool = True):
+        self._check(self._lib.pxd_enable_walk_timing(self._h, int(on)))
+
+    def walkStats(self):
+        a, b, c = C.c_uint64(), C.c_double(), C.c_double()
+        self._check(self._lib.pxd_walk_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), float(b.value), float(c.value)
+
+    # snake_case aliases
+    iterate_random_walk = iterateRandomWalk
+    compute_sweep_cuts = computeSweepCuts
+    cut_clusters = cutClusters
+    expander_decomposition = expanderDecomposition
+    get_final_partition = getFinalPartition
+    read_random_walk_values = readRandomWalkValues
+    download_degrees = downloadDegrees
+    download_labels = downloadLabels
+    download_graph = downloadGraph
+    download_active_edge_map = downloadActiveEdgeMap
+    download_partition = downloadPartition
+    read_sweep_cuts = readSweepCuts
+    get_num_cut_edges = getNumCutEdges
+    get_num_clusters = getNumClusters
+    update_labels = updateLabels

@@ -1,0 +1,183 @@
+// Device-side building blocks of the expander-decomposition engine for CDNA4
+// (gfx950, 64-lane wavefronts): exact fixed-point accumulation, ordered-float
+// keys and wave/block scans.  Included by pxd_engine.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pxd {
+
+constexpr int WAVE = 64;
+constexpr uint32_t INVALID = 0xFFFFFFFFu;
+constexpr double RW_STAY = 0.1;          // lib/core/definitions.h:48
+constexpr int FIX_SHIFT = 72;            // walk sums: signed fixed point, scale 2^72
+
+// error bits reported through the engine's status word
+constexpr uint32_t ERR_FIX_RANGE = 1u;   // |x/dInt| >= 2^31 or row longer than 2^24 arcs
+constexpr uint32_t ERR_ZERO_DEG = 4u;    // valid arc into a node with dInt == 0
+
+// ------------------------------------------------------------------ keys
+
+// Ordered image of a float (cuda/devRandomWalk.h:50-61, low half of packKey):
+// unsigned order equals float order with -0 < +0.
+__device__ __forceinline__ uint32_t ord_f32(float v) {
+    uint32_t i = __float_as_uint(v);
+    return i ^ ((i & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t o) {
+    uint32_t i = (o & 0x80000000u) ? (o ^ 0x80000000u) : ~o;
+    return __uint_as_float(i);
+}
+
+// ------------------------------------------------------------------ fixed point
+
+// Signed 128-bit accumulator as two 64-bit words (two's complement).
+struct Fix128 {
+    uint64_t lo;
+    uint64_t hi;
+};
+
+__device__ __forceinline__ void fix_add128(Fix128& a, uint64_t blo, uint64_t bhi) {
+    uint64_t lo = a.lo + blo;
+    a.hi = a.hi + bhi + (lo < a.lo ? 1ull : 0ull);
+    a.lo = lo;
+}
+
+// Adds trunc(y * 2^72) to the accumulator.  Terms with |y| >= 2^31 set `ovf`.
+__device__ __forceinline__ void fix_add_term(Fix128& a, double y, uint32_t& ovf) {
+    uint64_t b = (uint64_t)__double_as_longlong(y);
+    uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
+    if (E == 0) return;
+    uint64_t M = (b & ((1ull << 52) - 1)) | (1ull << 52);
+    int sh = (int)E - (1075 - FIX_SHIFT);
+    uint64_t qlo, qhi;
+    if (sh >= 0) {
+        if (sh > 50) { ovf = 1; return; }
+        qlo = M << sh;
+        qhi = sh ? (M >> (64 - sh)) : 0ull;
+    } else {
+        int r = -sh;
+        qlo = (r >= 53) ? 0ull : (M >> r);
+        qhi = 0ull;
+    }
+    if (b >> 63) {          // subtract: add two's complement of (qhi, qlo)
+        uint64_t nlo = ~qlo + 1ull;
+        uint64_t nhi = ~qhi + (nlo == 0ull ? 1ull : 0ull);
+        fix_add128(a, nlo, nhi);
+    } else {
+        fix_add128(a, qlo, qhi);
+    }
+}
+
+// Correctly rounded conversion of a value below 2^63 (exact halves, one rounding).
+__device__ __forceinline__ double u63_to_f64(uint64_t v) {
+    return ldexp((double)(uint32_t)(v >> 32), 32) + (double)(uint32_t)v;
+}
+
+// Accumulator -> double, one correct rounding, then exact scaling by 2^-72.
+__device__ __forceinline__ double fix_to_f64(Fix128 a) {
+    bool neg = (int64_t)a.hi < 0;
+    uint64_t lo = a.lo, hi = a.hi;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    double d;
+    if (hi == 0ull && (lo >> 63) == 0ull) {
+        d = u63_to_f64(lo);
+    } else {
+        int nb = hi ? (128 - __clzll((long long)hi)) : (64 - __clzll((long long)lo));
+        int k = nb - 63;                                   // 1..65
+        uint64_t top, rest;
+        if (k < 64) {
+            top = (lo >> k) | (hi << (64 - k));
+            rest = lo & ((1ull << k) - 1ull);
+        } else {
+            top = hi >> (k - 64);
+            rest = lo | (k > 64 ? (hi & ((1ull << (k - 64)) - 1ull)) : 0ull);
+        }
+        if (rest) top |= 1ull;                            // sticky bit
+        d = ldexp(u63_to_f64(top), k);
+    }
+    if (neg) d = -d;
+    return ldexp(d, -FIX_SHIFT);
+}
+
+// int64 -> double with one correct rounding
+__device__ __forceinline__ double i64_to_f64(int64_t v) {
+    return ldexp((double)(int32_t)(v >> 32), 32) + (double)(uint32_t)(uint64_t)v;
+}
+
+// ------------------------------------------------------------------ wave helpers
+
+template <int G>
+__device__ __forceinline__ Fix128 group_sum_fix(Fix128 a) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        uint64_t olo = __shfl_xor(a.lo, off, G);
+        uint64_t ohi = __shfl_xor(a.hi, off, G);
+        fix_add128(a, olo, ohi);
+    }
+    return a;
+}
+
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+// ballot restricted to the calling lane's group of G lanes (bit i = lane i of
+// the group), built from group-width exchanges so that groups of one wave may
+// run different trip counts
+template <int G>
+__device__ __forceinline__ uint64_t group_ballot(bool pred) {
+    uint64_t b = pred ? (1ull << (threadIdx.x & (G - 1))) : 0ull;
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) b |= __shfl_xor(b, off, G);
+    return b;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, G);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// Inclusive scan over the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        uint32_t t = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; `lds` holds >= BS/64 words.
+template <int BS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+    constexpr int NW = BS / WAVE;
+    uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == WAVE - 1) lds[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t w = lane < (uint32_t)NW ? lds[lane] : 0u;
+        w = wave_incl_scan(w);
+        if (lane < (uint32_t)NW) lds[lane] = w;
+    }
+    __syncthreads();
+    uint32_t before = wid ? lds[wid - 1] : 0u;
+    total = lds[NW - 1];
+    __syncthreads();
+    return before + inc - v;
+}
+
+}  // namespace pxd

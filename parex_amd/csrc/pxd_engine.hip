@@ -1,0 +1,993 @@
+// Host side of the MI355X expander-decomposition engine: device state, the
+// per-iteration launch sequence and the C ABI of include/parex_amd/pxd.h.
+//
+// One engine holds one graph or a block-diagonal batch of graphs.  Clusters
+// are contiguous ranges of the active-position list; all label arithmetic of
+// the reference (dense relabelling, deactivation labels, smallest label) is
+// carried per graph, so a graph's result does not depend on its batch-mates.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "parex_amd/pxd.h"
+#include "pxd_kernels.h"
+
+using namespace pxd;
+
+static thread_local std::string g_last_error;
+
+static int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPC(x)                                                                                   \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            return set_err(PXD_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),       \
+                           __FILE__, __LINE__);                                                   \
+    } while (0)
+
+#define CHK(x)                        \
+    do {                              \
+        int r_ = (x);                 \
+        if (r_ != 0) return r_;       \
+    } while (0)
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+static inline uint32_t grid_for(uint64_t n, uint32_t per_block) {
+    uint64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > (1u << 20)) g = 1u << 20;
+    return (uint32_t)g;
+}
+
+struct ClusterTable {
+    uint32_t *cs = nullptr, *cgraph = nullptr, *cmin = nullptr, *cflag = nullptr, *cvol = nullptr;
+    int32_t* clabel = nullptr;
+    ClusterCols cols() const { return ClusterCols{cs, cgraph, cmin, cflag, cvol, clabel}; }
+};
+
+struct pxd_engine {
+    pxd_params P{};
+    int dev = 0;
+    hipStream_t st = nullptr;
+    std::vector<void*> allocs;
+
+    // sizes
+    uint32_t G = 0, N = 0, A = 0;
+    uint64_t M_total = 0;
+    std::vector<uint32_t> h_gnode;    // G+1 node offsets
+    std::vector<uint32_t> h_garc;     // G+1 arc offsets
+    std::vector<uint32_t> h_gm;       // undirected edges per graph
+    bool inited = false;
+
+    // graph
+    uint32_t *d_row = nullptr, *d_deg = nullptr, *d_nbr = nullptr, *d_nbr0 = nullptr, *d_gnode = nullptr;
+    // per node
+    uint32_t* d_dint = nullptr;
+    int32_t* d_label = nullptr;
+    double* d_x[2] = {nullptr, nullptr};
+    double* d_y[2] = {nullptr, nullptr};
+    uint32_t* d_xkey = nullptr;
+    int32_t* d_ediff = nullptr;
+    uint32_t *d_root = nullptr, *d_part = nullptr, *d_stamp = nullptr;
+    uint8_t* d_alive = nullptr;
+    // positions
+    uint32_t *d_pnix[2] = {nullptr, nullptr}, *d_pkey[2] = {nullptr, nullptr};
+    uint32_t *d_pclu[2] = {nullptr, nullptr};
+    uint32_t* d_alist[2] = {nullptr, nullptr};
+    pxd_node_data* d_nd = nullptr;
+    // clusters
+    ClusterTable ct[2];
+    int ctc = 0;
+    uint32_t *d_coff = nullptr, *d_ckeep = nullptr, *d_kscan = nullptr, *d_gfirst = nullptr;
+    uint32_t *d_cmaxo = nullptr, *d_cmino = nullptr;
+    int32_t* d_cK = nullptr;
+    float* d_csp = nullptr;
+    double* d_cavg = nullptr;
+    unsigned long long *d_cbest = nullptr, *d_csum = nullptr;
+    // graphs
+    int32_t *d_gsmall = nullptr, *d_gsmall_next = nullptr, *d_giter = nullptr;
+    uint32_t *d_gsurv = nullptr, *d_gseen = nullptr;
+    // scratch
+    uint32_t *d_tA = nullptr, *d_tB = nullptr, *d_tC = nullptr;
+    uint32_t* d_cclist = nullptr;
+    uint32_t *d_lwave = nullptr, *d_lblock = nullptr, *d_llarge = nullptr;
+    uint32_t *d_lstart = nullptr, *d_llen = nullptr;
+    LargeSeg* d_ls = nullptr;
+    uint32_t *d_tseg = nullptr, *d_tidx = nullptr, *d_hist = nullptr, *d_hscan = nullptr;
+    uint32_t tiles_cap = 0, ls_cap = 0;
+    uint32_t *d_tail_ed = nullptr, *d_tail_vol = nullptr, *d_hasst = nullptr, *d_carry_ed = nullptr,
+             *d_carry_vol = nullptr;
+    std::vector<uint32_t*> lvl_sum, lvl_scan;
+    uint32_t* d_oneseg = nullptr;
+    uint32_t* d_gclu = nullptr;
+    unsigned long long* d_u64 = nullptr;
+    Counters* d_cnt = nullptr;
+    Counters hc{};
+
+    // host mirrors
+    int cur = 0, acur = 0, pcur = 0;
+    uint32_t na = 0, nc = 0, ncut = 0, ncc = 0, nalist = 0;
+    uint32_t iter1 = 0;
+    bool sweep_done = false;
+    int gs = 16;
+
+    // timing
+    double last_run_ms = 0.0;
+    bool walk_timing = false;
+    uint64_t walk_launches = 0;
+    double walk_ms = 0.0, walk_bytes = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~pxd_engine() {
+        release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        lvl_sum.clear();
+        lvl_scan.clear();
+        inited = false;
+    }
+
+    template <typename T>
+    int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess)
+            return set_err(PXD_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+
+    int sync() {
+        HIPC(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    int read_counters() {
+        HIPC(hipMemcpyAsync(&hc, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        CHK(sync());
+        if (hc.err & ERR_FIX_RANGE)
+            return set_err(PXD_ERR_NUMERIC, "walk value outside the exact fixed-point range (|x/dInt| >= 2^31)");
+        return 0;
+    }
+
+    int read_u32(const uint32_t* d, uint32_t* h) {
+        HIPC(hipMemcpyAsync(h, d, 4, hipMemcpyDeviceToHost, st));
+        return sync();
+    }
+
+    int zero_counter_fields() {
+        // counters other than the sticky error word
+        HIPC(hipMemsetAsync(((char*)d_cnt) + 4, 0, sizeof(Counters) - 4, st));
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- setup
+    int setup(uint32_t g, const uint32_t* ncnt, const uint32_t* ecnt, const uint32_t* const* rptr,
+              const uint32_t* const* eptr) {
+        release();
+        HIPC(hipSetDevice(dev));
+        G = g;
+        h_gnode.assign(G + 1, 0);
+        h_garc.assign(G + 1, 0);
+        h_gm.assign(G, 0);
+        uint64_t nn = 0, aa = 0;
+        for (uint32_t i = 0; i < G; ++i) {
+            h_gnode[i] = (uint32_t)nn;
+            h_garc[i] = (uint32_t)aa;
+            nn += ncnt[i];
+            aa += 2ull * ecnt[i];
+            h_gm[i] = ecnt[i];
+            if (rptr[i][ncnt[i]] != 2ull * ecnt[i] || rptr[i][0] != 0)
+                return set_err(PXD_ERR_ARG, "graph %u: ranges[n] must equal 2m and ranges[0] == 0", i);
+        }
+        if (nn >= 0x7FFFFFFFull || aa >= 0xFFFFFFFFull)
+            return set_err(PXD_ERR_ARG, "batch too large for 32-bit indices (%llu nodes, %llu arcs)",
+                           (unsigned long long)nn, (unsigned long long)aa);
+        h_gnode[G] = (uint32_t)nn;
+        h_garc[G] = (uint32_t)aa;
+        N = (uint32_t)nn;
+        A = (uint32_t)aa;
+        M_total = aa / 2;
+
+        // host CSR of the whole batch (global node ids)
+        std::vector<uint32_t> row(N + 1);
+        std::vector<uint32_t> nbr(A);
+        for (uint32_t i = 0; i < G; ++i) {
+            const uint32_t nb = h_gnode[i], ab = h_garc[i];
+            for (uint32_t u = 0; u < ncnt[i]; ++u) row[nb + u] = ab + rptr[i][u];
+            for (uint64_t j = 0; j < 2ull * ecnt[i]; ++j) {
+                const uint32_t v = eptr[i][j];
+                if (v >= ncnt[i]) return set_err(PXD_ERR_ARG, "graph %u: arc %llu points to %u >= n", i,
+                                                 (unsigned long long)j, v);
+                nbr[ab + j] = nb + v;
+            }
+            for (uint32_t u = 0; u < ncnt[i]; ++u)
+                if (rptr[i][u + 1] < rptr[i][u]) return set_err(PXD_ERR_ARG, "graph %u: ranges decrease at %u", i, u);
+        }
+        row[N] = A;
+
+        const size_t n1 = (size_t)N + 1;
+        CHK(alloc(&d_row, n1));
+        CHK(alloc(&d_deg, N));
+        CHK(alloc(&d_nbr, A));
+        if (P.keep_original) CHK(alloc(&d_nbr0, A));
+        CHK(alloc(&d_gnode, G + 1));
+        CHK(alloc(&d_dint, N));
+        CHK(alloc(&d_label, N));
+        for (int b = 0; b < 2; ++b) {
+            CHK(alloc(&d_x[b], N));
+            CHK(alloc(&d_y[b], N));
+            CHK(alloc(&d_pnix[b], N));
+            CHK(alloc(&d_pkey[b], N));
+            CHK(alloc(&d_pclu[b], N));
+            CHK(alloc(&d_alist[b], N));
+            CHK(alloc(&ct[b].cs, n1 + 1));
+            CHK(alloc(&ct[b].cgraph, n1));
+            CHK(alloc(&ct[b].cmin, n1));
+            CHK(alloc(&ct[b].cflag, n1));
+            CHK(alloc(&ct[b].cvol, n1));
+            CHK(alloc(&ct[b].clabel, n1));
+        }
+        CHK(alloc(&d_xkey, N));
+        CHK(alloc(&d_ediff, N));
+        CHK(alloc(&d_root, N));
+        CHK(alloc(&d_part, N));
+        CHK(alloc(&d_stamp, N));
+        CHK(alloc(&d_alive, N));
+        if (P.record_node_data) CHK(alloc(&d_nd, N));
+        CHK(alloc(&d_coff, n1));
+        CHK(alloc(&d_ckeep, n1));
+        CHK(alloc(&d_kscan, n1 + 1));
+        CHK(alloc(&d_cmaxo, n1));
+        CHK(alloc(&d_cmino, n1));
+        CHK(alloc(&d_cK, n1));
+        CHK(alloc(&d_csp, n1));
+        CHK(alloc(&d_cavg, n1));
+        CHK(alloc(&d_cbest, n1));
+        CHK(alloc(&d_csum, n1));
+        CHK(alloc(&d_gfirst, G + 1));
+        CHK(alloc(&d_gsmall, G));
+        CHK(alloc(&d_gsmall_next, G));
+        CHK(alloc(&d_giter, G));
+        CHK(alloc(&d_gsurv, G));
+        CHK(alloc(&d_gseen, G));
+        CHK(alloc(&d_tA, n1 + 1));
+        CHK(alloc(&d_tB, n1 + 1));
+        CHK(alloc(&d_tC, n1 + 1));
+        CHK(alloc(&d_cclist, N));
+        CHK(alloc(&d_lwave, n1));
+        CHK(alloc(&d_lblock, n1));
+        CHK(alloc(&d_llarge, n1));
+        ls_cap = N / TILE + 2;
+        tiles_cap = 2 * (N / TILE) + 4;
+        CHK(alloc(&d_lstart, ls_cap));
+        CHK(alloc(&d_llen, ls_cap));
+        CHK(alloc(&d_ls, ls_cap));
+        CHK(alloc(&d_tseg, tiles_cap));
+        CHK(alloc(&d_tidx, tiles_cap));
+        CHK(alloc(&d_hist, (size_t)tiles_cap * 256 + 1));
+        CHK(alloc(&d_hscan, (size_t)tiles_cap * 256 + 2));
+        const uint32_t nbk = cdiv(n1, BS * SCAN_ITEMS) + 1;
+        CHK(alloc(&d_tail_ed, nbk));
+        CHK(alloc(&d_tail_vol, nbk));
+        CHK(alloc(&d_hasst, nbk));
+        CHK(alloc(&d_carry_ed, nbk));
+        CHK(alloc(&d_carry_vol, nbk));
+        {
+            uint64_t len = std::max<uint64_t>(n1 + 1, (uint64_t)tiles_cap * 256 + 2);
+            while (true) {
+                uint32_t nb = cdiv(len, BS * SCAN_ITEMS);
+                uint32_t *s = nullptr, *c = nullptr;
+                CHK(alloc(&s, nb + 1));
+                CHK(alloc(&c, nb + 2));
+                lvl_sum.push_back(s);
+                lvl_scan.push_back(c);
+                if (nb <= 1) break;
+                len = nb;
+            }
+        }
+        CHK(alloc(&d_oneseg, 2));
+        CHK(alloc(&d_gclu, G));
+        CHK(alloc(&d_u64, 1));
+        CHK(alloc(&d_cnt, 1));
+
+        HIPC(hipMemcpyAsync(d_row, row.data(), n1 * 4, hipMemcpyHostToDevice, st));
+        if (A) HIPC(hipMemcpyAsync(d_nbr, nbr.data(), (size_t)A * 4, hipMemcpyHostToDevice, st));
+        if (P.keep_original && A)
+            HIPC(hipMemcpyAsync(d_nbr0, d_nbr, (size_t)A * 4, hipMemcpyDeviceToDevice, st));
+        HIPC(hipMemcpyAsync(d_gnode, h_gnode.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
+        CHK(sync());  // host vectors go out of scope
+        return init_state();
+    }
+
+    // initial clusters: one per non-empty graph (label 0, connectivity unknown)
+    int init_state() {
+        std::vector<uint32_t> cs, cg, cmin, cvol, gclu(G, 0);
+        for (uint32_t g = 0; g < G; ++g) {
+            gclu[g] = (uint32_t)cs.size();
+            if (h_gnode[g + 1] == h_gnode[g]) continue;
+            cs.push_back(h_gnode[g]);
+            cg.push_back(g);
+            cmin.push_back(h_gnode[g]);
+            cvol.push_back(h_garc[g + 1] - h_garc[g]);
+        }
+        nc = (uint32_t)cs.size();
+        cs.push_back(N);
+        std::vector<uint32_t> zeros(nc + 1, 0);
+        ctc = 0;
+        ClusterTable& T = ct[0];
+        HIPC(hipMemcpyAsync(T.cs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.cgraph, cg.data(), nc * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.cmin, cmin.data(), nc * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.cvol, cvol.data(), nc * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.cflag, zeros.data(), nc * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(T.clabel, zeros.data(), nc * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(d_gclu, gclu.data(), G * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemsetAsync(d_gsmall, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gsmall_next, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_giter, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gsurv, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gseen, 0, G * 4, st));
+        cur = acur = pcur = 0;
+        if (N) {
+            k_init_nodes<<<grid_for(N, BS), BS, 0, st>>>(N, G, d_gnode, d_row, P.seed, d_deg, d_dint, d_x[0], d_y[0],
+                                                          d_xkey, d_label, d_alive, d_stamp, d_pnix[0], d_alist[0],
+                                                          d_pclu[0], d_gclu);
+            HIPC(hipGetLastError());
+        }
+        CHK(sync());
+        na = N;
+        nalist = N;
+        ncut = 0;
+        ncc = 0;
+        iter1 = 0;
+        sweep_done = false;
+        const double avg = N ? (double)A / (double)N : 0.0;
+        gs = avg <= 6.0 ? 4 : avg <= 12.0 ? 8 : avg <= 24.0 ? 16 : avg <= 48.0 ? 32 : 64;
+        inited = true;
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- scan
+    int scan_level(const uint32_t* in, uint32_t* out, uint32_t n, size_t lvl) {
+        const uint32_t nb = cdiv(n, BS * SCAN_ITEMS);
+        if (nb <= 1) {
+            k_scan_down<<<1, BS, 0, st>>>(n, in, out, nullptr);
+            HIPC(hipGetLastError());
+            return 0;
+        }
+        if (lvl >= lvl_sum.size()) return set_err(PXD_ERR_STATE, "scan too deep");
+        k_scan_up<<<nb, BS, 0, st>>>(n, in, lvl_sum[lvl]);
+        HIPC(hipGetLastError());
+        CHK(scan_level(lvl_sum[lvl], lvl_scan[lvl], nb, lvl + 1));
+        k_scan_down<<<nb, BS, 0, st>>>(n, in, out, lvl_scan[lvl]);
+        HIPC(hipGetLastError());
+        return 0;
+    }
+    // out[0..n) exclusive prefix sums, out[n] = total
+    int scan(const uint32_t* in, uint32_t* out, uint32_t n) {
+        if (n == 0) {
+            k_fill_u32<<<1, 64, 0, st>>>(1, out, 0u);
+            HIPC(hipGetLastError());
+            return 0;
+        }
+        return scan_level(in, out, n, 0);
+    }
+
+    // ---------------------------------------------------------------- segmented sort
+    // Stable sort of (pkey[0], pnix[0]) inside every segment [segs[c], segs[c+1]).
+    int seg_sort(const uint32_t* segs, uint32_t nseg) {
+        if (nseg == 0) return 0;
+        HIPC(hipMemsetAsync(&d_cnt->nwave, 0, 3 * sizeof(uint32_t), st));
+        k_classify<<<grid_for(nseg, BS), BS, 0, st>>>(nseg, segs, d_lwave, d_lblock, d_llarge, d_cnt);
+        HIPC(hipGetLastError());
+        CHK(read_counters());
+        const uint32_t nwave = hc.nwave, nblock = hc.nblock, nlarge = hc.nlarge;
+        if (nwave) {
+            k_sort_wave<<<cdiv(nwave, BS / WAVE), BS, 0, st>>>(nwave, d_lwave, segs, d_pkey[0], d_pnix[0]);
+            HIPC(hipGetLastError());
+        }
+        if (nblock) {
+            k_sort_block<<<nblock, BS, 0, st>>>(nblock, d_lblock, segs, d_pkey[0], d_pnix[0]);
+            HIPC(hipGetLastError());
+        }
+        if (nlarge) {
+            if (nlarge > ls_cap) return set_err(PXD_ERR_STATE, "too many large segments");
+            // start and length of each large segment
+            k_gather<<<grid_for(nlarge, BS), BS, 0, st>>>(nlarge, d_llarge, segs, d_lstart);
+            HIPC(hipGetLastError());
+            std::vector<uint32_t> lst(nlarge), lid(nlarge);
+            HIPC(hipMemcpyAsync(lst.data(), d_lstart, nlarge * 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(lid.data(), d_llarge, nlarge * 4, hipMemcpyDeviceToHost, st));
+            // lengths need segs[c+1]: gather from segs + 1
+            k_gather<<<grid_for(nlarge, BS), BS, 0, st>>>(nlarge, d_llarge, segs + 1, d_llen);
+            HIPC(hipGetLastError());
+            std::vector<uint32_t> lend(nlarge);
+            HIPC(hipMemcpyAsync(lend.data(), d_llen, nlarge * 4, hipMemcpyDeviceToHost, st));
+            CHK(sync());
+            std::vector<LargeSeg> ls(nlarge);
+            std::vector<uint32_t> tseg, tidx;
+            uint32_t tiles = 0;
+            for (uint32_t i = 0; i < nlarge; ++i) {
+                ls[i].start = lst[i];
+                ls[i].len = lend[i] - lst[i];
+                ls[i].ntiles = cdiv(ls[i].len, TILE);
+                ls[i].hbase = tiles * 256u;
+                for (uint32_t t = 0; t < ls[i].ntiles; ++t) { tseg.push_back(i); tidx.push_back(t); }
+                tiles += ls[i].ntiles;
+            }
+            if (tiles > tiles_cap) return set_err(PXD_ERR_STATE, "tile capacity exceeded");
+            HIPC(hipMemcpyAsync(d_ls, ls.data(), nlarge * sizeof(LargeSeg), hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_tseg, tseg.data(), tiles * 4, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_tidx, tidx.data(), tiles * 4, hipMemcpyHostToDevice, st));
+            int src = 0;
+            for (int pass = 0; pass < 4; ++pass) {
+                const int sh = 8 * pass;
+                k_tile_hist<<<tiles, BS, 0, st>>>(tiles, d_tseg, d_tidx, d_ls, d_pkey[src], sh, d_hist);
+                HIPC(hipGetLastError());
+                CHK(scan(d_hist, d_hscan, tiles * 256u));
+                k_tile_scatter<<<tiles, BS, 0, st>>>(tiles, d_tseg, d_tidx, d_ls, d_hscan, d_pkey[src], d_pnix[src], sh,
+                                                     d_pkey[src ^ 1], d_pnix[src ^ 1]);
+                HIPC(hipGetLastError());
+                src ^= 1;
+            }
+            CHK(sync());  // host vectors above go out of scope
+        }
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- steps
+    template <int GSZ>
+    void launch_walk() {
+        const uint32_t groups_per_block = BS / GSZ;
+        k_walk<GSZ><<<grid_for(nalist, groups_per_block), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
+                                                                         d_y[cur], d_x[cur], d_x[cur ^ 1],
+                                                                         d_y[cur ^ 1], d_xkey, d_cnt);
+    }
+    template <int GSZ>
+    void launch_contrib() {
+        k_contrib<GSZ><<<grid_for(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
+                                                                     d_xkey, d_ediff);
+    }
+    template <int GSZ>
+    void launch_hook(uint32_t nl) {
+        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root,
+                                                                d_cnt);
+    }
+    template <int GSZ>
+    void launch_disable(uint32_t nl) {
+        k_disable<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
+                                                                d_x[cur], d_y[cur]);
+    }
+
+#define PXD_DISPATCH_GS(fn, ...)                          \
+    switch (gs) {                                         \
+        case 4: fn<4>(__VA_ARGS__); break;                \
+        case 8: fn<8>(__VA_ARGS__); break;                \
+        case 16: fn<16>(__VA_ARGS__); break;              \
+        case 32: fn<32>(__VA_ARGS__); break;              \
+        default: fn<64>(__VA_ARGS__); break;              \
+    }
+
+    int walk() {
+        if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+        if (nalist) {
+            if (walk_timing) HIPC(hipEventRecord(ev0, st));
+            PXD_DISPATCH_GS(launch_walk);
+            HIPC(hipGetLastError());
+            if (walk_timing) {
+                HIPC(hipEventRecord(ev1, st));
+                HIPC(hipEventSynchronize(ev1));
+                float ms = 0.f;
+                HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+                // valid arcs of active nodes: sum of dInt over the node list
+                HIPC(hipMemsetAsync(d_u64, 0, 8, st));
+                // (dInt of inactive nodes is stale; sum only over the active list)
+                k_gather<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, d_tC);
+                k_sum_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tC, d_u64);
+                unsigned long long arcs = 0;
+                HIPC(hipMemcpyAsync(&arcs, d_u64, 8, hipMemcpyDeviceToHost, st));
+                CHK(sync());
+                walk_launches++;
+                walk_ms += ms;
+                walk_bytes += 12.0 * (double)arcs + 32.0 * (double)nalist;
+            }
+            cur ^= 1;                 // the walk wrote x' and y' = x'/dInt into the other buffers
+        } else {
+            cur ^= 1;
+        }
+        sweep_done = false;
+        return 0;
+    }
+
+    int sweep() {
+        if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+        ClusterTable& T = ct[ctc];
+        ncut = 0;
+        ncc = 0;
+        if (na == 0 || nc == 0) { sweep_done = true; return 0; }
+        // keys in position order, then stable sort inside every cluster
+        k_gather<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
+        HIPC(hipGetLastError());
+        CHK(seg_sort(T.cs, nc));
+        PXD_DISPATCH_GS(launch_contrib);
+        HIPC(hipGetLastError());
+        k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, nullptr, nullptr, nullptr, d_cbest);
+        const uint32_t nb = cdiv(na, BS * SCAN_ITEMS);
+        k_sweep_a<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, d_ediff, d_dint, d_tail_ed, d_tail_vol, d_hasst);
+        HIPC(hipGetLastError());
+        k_sweep_b<<<1, 1024, 0, st>>>(nb, d_tail_ed, d_tail_vol, d_hasst, d_carry_ed, d_carry_vol);
+        HIPC(hipGetLastError());
+        k_sweep_c<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, T.cvol, d_ediff, d_dint, d_carry_ed, d_carry_vol,
+                                     d_cbest, P.record_node_data ? d_nd : nullptr);
+        HIPC(hipGetLastError());
+        CHK(zero_counter_fields());
+        k_decide<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cbest, (double)P.sc_threshold, T.cflag, d_coff, d_csp, d_cnt);
+        HIPC(hipGetLastError());
+        CHK(read_counters());
+        ncut = hc.ncut;
+        ncc = hc.ncc;
+        sweep_done = true;
+        return 0;
+    }
+
+    int cut() {
+        if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+        if (!sweep_done) return set_err(PXD_ERR_ARG, "cutClusters needs a preceding computeSweepCuts");
+        sweep_done = false;
+        ++iter1;
+        if (na == 0 || nc == 0) return 0;
+        ClusterTable* T = &ct[ctc];
+        uint32_t nl = 0;
+        if (ncc > 0) {
+            CHK(zero_counter_fields());
+            k_mark<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
+                                                    d_root, d_stamp, d_cclist, d_cnt);
+            HIPC(hipGetLastError());
+            CHK(read_counters());
+            nl = hc.cclist_n;
+            CHK(connected_components(nl));
+            // regroup: stable sort of all positions by component / cluster minimum
+            k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
+            HIPC(hipGetLastError());
+            const uint32_t one[2] = {0u, na};
+            HIPC(hipMemcpyAsync(d_oneseg, one, 8, hipMemcpyHostToDevice, st));
+            CHK(seg_sort(d_oneseg, 1));
+            k_run_flags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pkey[0], d_tA);
+            HIPC(hipGetLastError());
+            CHK(scan(d_tA, d_tB, na));
+            k_build_clusters<<<grid_for(na, BS), BS, 0, st>>>(na, G, d_gnode, d_pkey[0], d_tA, d_tB, T->cs, T->cmin,
+                                                              T->cgraph, T->cflag, d_pclu[0]);
+            HIPC(hipGetLastError());
+            CHK(read_u32(d_tB + na, &nc));
+        }
+        // statistics of the (possibly new) clusters
+        k_graph_first<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst);
+        k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cmaxo, d_cmino, d_csum, nullptr);
+        k_stats_minmax<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cmaxo, d_cmino);
+        k_stats_scale<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK);
+        k_stats_sum<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
+        k_stats_final<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold,
+                                                       d_ckeep, d_cavg);
+        HIPC(hipGetLastError());
+        CHK(scan(d_ckeep, d_kscan, nc));
+        k_labels<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst, d_ckeep, d_kscan, d_gsmall, d_gsmall_next,
+                                                  d_gsurv, T->clabel);
+        k_mark_seen_launch(nc, T->cgraph);
+        k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter, iter1);
+        k_apply<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_ckeep, T->clabel, d_cavg, d_dint, d_label,
+                                                 d_x[cur], d_y[cur], d_alive);
+        HIPC(hipGetLastError());
+        // drop deactivated clusters from the position list, the cluster table and the node list
+        k_pos_keepflags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pclu[0], d_ckeep, d_tA);
+        CHK(scan(d_tA, d_tB, na));
+        k_pos_compact<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_pclu[0], d_kscan, d_pnix[1],
+                                                       d_pclu[1]);
+        HIPC(hipGetLastError());
+        uint32_t na_new = 0, nc_new = 0;
+        CHK(read_u32(d_tB + na, &na_new));
+        CHK(read_u32(d_kscan + nc, &nc_new));
+        ClusterTable& T2 = ct[ctc ^ 1];
+        if (nc_new) {
+            k_clu_compact<<<grid_for(nc, BS), BS, 0, st>>>(nc, na_new, d_ckeep, d_kscan, d_tB, T->cols(), T2.cols());
+            HIPC(hipGetLastError());
+        }
+        std::swap(d_pnix[0], d_pnix[1]);
+        std::swap(d_pclu[0], d_pclu[1]);
+        ctc ^= 1;
+        if (na_new != na) {
+            k_alive_flags<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_alive, d_tA);
+            CHK(scan(d_tA, d_tC, nalist));
+            k_compact_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tA, d_tC, d_alist[acur], d_alist[acur ^ 1]);
+            HIPC(hipGetLastError());
+            acur ^= 1;
+            nalist = na_new;
+        }
+        na = na_new;
+        nc = nc_new;
+        if (nl) {
+            PXD_DISPATCH_GS(launch_disable, nl);
+            HIPC(hipGetLastError());
+        }
+        if (nc) {
+            ClusterTable& T3 = ct[ctc];
+            k_fill_u32<<<grid_for(nc, BS), BS, 0, st>>>(nc, T3.cvol, 0u);
+            k_volume<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
+            HIPC(hipGetLastError());
+        }
+        return 0;
+    }
+
+    void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph);
+
+    int connected_components(uint32_t nl) {
+        if (nl == 0) return 0;
+        for (int round = 0;; ++round) {
+            HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
+            PXD_DISPATCH_GS(launch_hook, nl);
+            HIPC(hipGetLastError());
+            for (;;) {
+                HIPC(hipMemsetAsync(&d_cnt->jump_changed, 0, 4, st));
+                for (int j = 0; j < 4; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
+                HIPC(hipGetLastError());
+                CHK(read_counters());
+                if (!hc.jump_changed) break;
+            }
+            if (!hc.hook_changed) break;
+            if (round > 100000) return set_err(PXD_ERR_STATE, "connected components did not converge");
+        }
+        return 0;
+    }
+
+    int run(int32_t* iterations) {
+        if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+        HIPC(hipEventRecord(ev0, st));
+        const uint32_t maxit = P.max_iter;
+        uint32_t i = 0;
+        while (i++ < maxit && nc > 0 && na > 0) {
+            CHK(walk());
+            CHK(sweep());
+            if (na == 0) break;
+            CHK(cut());
+        }
+        HIPC(hipEventRecord(ev1, st));
+        HIPC(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+        last_run_ms = ms;
+        // graphs still unfinished (max_iter) or never started report the final counter
+        std::vector<int32_t> gi(G);
+        if (G) HIPC(hipMemcpy(gi.data(), d_giter, G * 4, hipMemcpyDeviceToHost));
+        int32_t mx = 0;
+        for (uint32_t g = 0; g < G; ++g) {
+            if (gi[g] == 0) gi[g] = (int32_t)i;
+            mx = std::max(mx, gi[g]);
+        }
+        if (G) HIPC(hipMemcpy(d_giter, gi.data(), G * 4, hipMemcpyHostToDevice));
+        if (iterations) *iterations = G ? mx : (int32_t)i;
+        return 0;
+    }
+};
+
+__global__ void k_mark_seen(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gseen) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) gseen[cgraph[c]] = 1u;
+}
+
+__global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32_t* __restrict__ dint,
+                          double* __restrict__ x, double* __restrict__ y, uint32_t* __restrict__ xkey) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        const double a = v[u];
+        x[u] = a;
+        const uint32_t d = dint[u];
+        y[u] = d ? a / (double)d : 0.0;
+        xkey[u] = ord_f32((float)a);
+    }
+}
+
+void pxd_engine::k_mark_seen_launch(uint32_t n, const uint32_t* cgraph) {
+    k_mark_seen<<<grid_for(n, BS), BS, 0, st>>>(n, cgraph, d_gseen);
+}
+
+// =================================================================== C ABI
+
+extern "C" {
+
+void pxd_default_params(pxd_params* p) {
+    if (!p) return;
+    p->seed = 6582u;
+    p->sc_threshold = 0.3f;
+    p->rw_threshold = 1e-4f;
+    p->max_iter = 200000u;
+    p->keep_original = 1u;
+    p->record_node_data = 1u;
+    p->device = 0;
+}
+
+const char* pxd_last_error(void) { return g_last_error.c_str(); }
+
+int pxd_create(const pxd_params* params, pxd_engine** out) {
+    if (!out) return set_err(PXD_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    pxd_engine* e = new (std::nothrow) pxd_engine();
+    if (!e) return set_err(PXD_ERR_OOM, "host allocation failed");
+    if (params) e->P = *params; else pxd_default_params(&e->P);
+    e->dev = e->P.device;
+    int ndev = 0;
+    hipError_t err = hipGetDeviceCount(&ndev);
+    if (err != hipSuccess || ndev <= e->dev) {
+        delete e;
+        return set_err(PXD_ERR_HIP, "no HIP device %d available (%s)", params ? params->device : 0,
+                       hipGetErrorString(err));
+    }
+    if (hipSetDevice(e->dev) != hipSuccess || hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        delete e;
+        return set_err(PXD_ERR_HIP, "stream/event creation failed");
+    }
+    *out = e;
+    return 0;
+}
+
+int pxd_destroy(pxd_engine* e) {
+    if (e) {
+        (void)hipSetDevice(e->dev);
+        delete e;
+    }
+    return 0;
+}
+
+int pxd_initialize(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges) {
+    if (!e || !ranges || (m && !edges)) return set_err(PXD_ERR_ARG, "null argument");
+    const uint32_t* r[1] = {ranges};
+    const uint32_t* ed[1] = {edges};
+    return e->setup(1, &n, &m, r, ed);
+}
+
+int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* node_counts, const uint32_t* edge_counts,
+                         const uint32_t* const* ranges_ptrs, const uint32_t* const* edges_ptrs) {
+    if (!e || !node_counts || !edge_counts || !ranges_ptrs || !edges_ptrs || num_graphs == 0)
+        return set_err(PXD_ERR_ARG, "null argument or empty batch");
+    return e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
+}
+
+int pxd_iterate_random_walk(pxd_engine* e) { return e ? e->walk() : set_err(PXD_ERR_ARG, "null engine"); }
+int pxd_compute_sweep_cuts(pxd_engine* e) { return e ? e->sweep() : set_err(PXD_ERR_ARG, "null engine"); }
+int pxd_cut_clusters(pxd_engine* e) { return e ? e->cut() : set_err(PXD_ERR_ARG, "null engine"); }
+
+int pxd_expander_decomposition(pxd_engine* e, int32_t* iterations) {
+    return e ? e->run(iterations) : set_err(PXD_ERR_ARG, "null engine");
+}
+
+int pxd_graph_iterations(pxd_engine* e, int32_t* out, uint32_t num_graphs) {
+    if (!e || !e->inited || !out || num_graphs < e->G) return set_err(PXD_ERR_ARG, "bad arguments");
+    HIPC(hipSetDevice(e->dev));
+    HIPC(hipMemcpy(out, e->d_giter, e->G * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int require(pxd_engine* e) {
+    if (!e || !e->inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+    HIPC(hipSetDevice(e->dev));
+    HIPC(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+int pxd_get_final_partition(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters) {
+    CHK(require(e));
+    if ((e->N && !ids_out) || !num_clusters) return set_err(PXD_ERR_ARG, "null output");
+    std::vector<int32_t> lab(e->N);
+    if (e->N) HIPC(hipMemcpy(lab.data(), e->d_label, e->N * 4, hipMemcpyDeviceToHost));
+    // cuda/devPartition.h:798-823 per graph: sort, unique, rank
+    for (uint32_t g = 0; g < e->G; ++g) {
+        const uint32_t b = e->h_gnode[g], en = e->h_gnode[g + 1];
+        std::vector<int32_t> u(lab.begin() + b, lab.begin() + en);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (uint32_t i = b; i < en; ++i)
+            ids_out[i] = (int32_t)(std::lower_bound(u.begin(), u.end(), lab[i]) - u.begin());
+        num_clusters[g] = (int32_t)u.size();
+    }
+    return 0;
+}
+
+int pxd_read_random_walk_values(pxd_engine* e, float* out) {
+    CHK(require(e));
+    std::vector<double> v(e->N);
+    if (e->N) HIPC(hipMemcpy(v.data(), e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < e->N; ++i) out[i] = (float)v[i];
+    return 0;
+}
+
+int pxd_read_random_walk_values_f64(pxd_engine* e, double* out) {
+    CHK(require(e));
+    if (e->N) HIPC(hipMemcpy(out, e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pxd_set_random_walk_values_f64(pxd_engine* e, const double* values) {
+    CHK(require(e));
+    if (!e->N) return 0;
+    double* tmp = nullptr;
+    HIPC(hipMalloc(&tmp, (size_t)e->N * 8));
+    HIPC(hipMemcpy(tmp, values, (size_t)e->N * 8, hipMemcpyHostToDevice));
+    k_set_all<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, tmp, e->d_dint, e->d_x[e->cur], e->d_y[e->cur], e->d_xkey);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(e->st));
+    HIPC(hipFree(tmp));
+    return 0;
+}
+
+int pxd_download_degrees(pxd_engine* e, uint32_t* out) {
+    CHK(require(e));
+    if (e->N) HIPC(hipMemcpy(out, e->d_dint, e->N * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pxd_download_labels(pxd_engine* e, int32_t* out) {
+    CHK(require(e));
+    if (e->N) HIPC(hipMemcpy(out, e->d_label, e->N * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// rebuild the reference's in-place INVALID_EDGE view: the valid prefix of a
+// row is an order-preserving subsequence of the original row
+static int edge_view(pxd_engine* e, std::vector<uint32_t>& row, std::vector<uint32_t>& orig,
+                     std::vector<uint8_t>& valid) {
+    if (!e->P.keep_original) return set_err(PXD_ERR_ARG, "engine created with keep_original = 0");
+    row.resize(e->N + 1);
+    orig.resize(e->A);
+    std::vector<uint32_t> cur(e->A), dint(e->N);
+    HIPC(hipMemcpy(row.data(), e->d_row, (e->N + 1) * 4, hipMemcpyDeviceToHost));
+    if (e->A) {
+        HIPC(hipMemcpy(orig.data(), e->d_nbr0, e->A * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(cur.data(), e->d_nbr, e->A * 4, hipMemcpyDeviceToHost));
+    }
+    if (e->N) HIPC(hipMemcpy(dint.data(), e->d_dint, e->N * 4, hipMemcpyDeviceToHost));
+    valid.assign(e->A, 0);
+    for (uint32_t u = 0; u < e->N; ++u) {
+        uint32_t p = row[u];
+        const uint32_t pe = row[u] + dint[u];
+        for (uint32_t j = row[u]; j < row[u + 1] && p < pe; ++j)
+            if (orig[j] == cur[p]) { valid[j] = 1; ++p; }
+        if (p != pe) return set_err(PXD_ERR_STATE, "valid prefix of node %u is not a subsequence of its row", u);
+    }
+    return 0;
+}
+
+int pxd_download_graph(pxd_engine* e, uint32_t* ranges_out, uint32_t* edges_out) {
+    CHK(require(e));
+    std::vector<uint32_t> row, orig;
+    std::vector<uint8_t> valid;
+    CHK(edge_view(e, row, orig, valid));
+    for (uint32_t g = 0; g < e->G; ++g) {
+        const uint32_t nb = e->h_gnode[g], ab = e->h_garc[g];
+        for (uint32_t u = nb; u <= e->h_gnode[g + 1]; ++u) ranges_out[u] = row[u] - ab;
+        for (uint32_t j = ab; j < e->h_garc[g + 1]; ++j) edges_out[j] = valid[j] ? orig[j] - nb : PXD_INVALID_EDGE;
+    }
+    return 0;
+}
+
+int pxd_download_active_edge_map(pxd_engine* e, int32_t* out) {
+    CHK(require(e));
+    std::vector<uint32_t> row, orig;
+    std::vector<uint8_t> valid;
+    CHK(edge_view(e, row, orig, valid));
+    for (uint32_t j = 0; j < e->A; ++j) out[j] = valid[j];
+    return 0;
+}
+
+int pxd_download_partition(pxd_engine* e, pxd_node_data* out, uint32_t capacity, uint32_t* count) {
+    CHK(require(e));
+    if (!e->P.record_node_data) return set_err(PXD_ERR_ARG, "engine created with record_node_data = 0");
+    const uint32_t c = std::min(capacity, e->sweep_done ? e->na : 0u);
+    if (c) HIPC(hipMemcpy(out, e->d_nd, c * sizeof(pxd_node_data), hipMemcpyDeviceToHost));
+    if (count) *count = c;
+    return 0;
+}
+
+int pxd_read_sweep_cuts(pxd_engine* e, int32_t* cluster_ids, pxd_sweep_cut* cuts, uint32_t capacity, uint32_t* count) {
+    CHK(require(e));
+    const uint32_t c = std::min(capacity, e->nc);
+    ClusterTable& T = e->ct[e->ctc];
+    std::vector<int32_t> lab(c);
+    std::vector<float> sp(c);
+    std::vector<uint32_t> off(c);
+    if (c) {
+        HIPC(hipMemcpy(lab.data(), T.clabel, c * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(sp.data(), e->d_csp, c * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(off.data(), e->d_coff, c * 4, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < c; ++i) {
+        if (cluster_ids) cluster_ids[i] = lab[i];
+        if (cuts) { cuts[i].clusterId = lab[i]; cuts[i].sparsity = sp[i]; cuts[i].offset = off[i]; }
+    }
+    if (count) *count = c;
+    return 0;
+}
+
+int pxd_get_num_cut_edges(pxd_engine* e, uint32_t* out) {
+    CHK(require(e));
+    HIPC(hipMemsetAsync(e->d_u64, 0, 8, e->st));
+    if (e->N) k_sum_u32<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, e->d_dint, e->d_u64);
+    HIPC(hipGetLastError());
+    unsigned long long s = 0;
+    HIPC(hipMemcpyAsync(&s, e->d_u64, 8, hipMemcpyDeviceToHost, e->st));
+    HIPC(hipStreamSynchronize(e->st));
+    *out = (uint32_t)(e->M_total - s / 2);
+    return 0;
+}
+
+int pxd_download_active_nodes(pxd_engine* e, pxd_labeled_node* out, uint32_t capacity, uint32_t* count) {
+    CHK(require(e));
+    const uint32_t c = std::min(capacity, e->na);
+    std::vector<uint32_t> nix(c), clu(c);
+    std::vector<int32_t> lab(e->nc);
+    if (c) {
+        HIPC(hipMemcpy(nix.data(), e->d_pnix[0], c * 4, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(clu.data(), e->d_pclu[0], c * 4, hipMemcpyDeviceToHost));
+    }
+    if (e->nc) HIPC(hipMemcpy(lab.data(), e->ct[e->ctc].clabel, e->nc * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < c; ++i) {
+        out[i].nix = nix[i] - e->h_gnode[0];
+        out[i].clusterId = clu[i] < e->nc ? lab[clu[i]] : -1;
+    }
+    if (count) *count = c;
+    return 0;
+}
+
+int pxd_get_counts(pxd_engine* e, uint32_t* num_active_nodes, uint32_t* num_active_clusters,
+                   uint32_t* num_clusters_with_cut) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    if (num_active_nodes) *num_active_nodes = e->na;
+    if (num_active_clusters) *num_active_clusters = e->nc;
+    if (num_clusters_with_cut) *num_clusters_with_cut = e->ncut;
+    return 0;
+}
+
+int pxd_last_run_ms(pxd_engine* e, double* ms) {
+    if (!e || !ms) return set_err(PXD_ERR_ARG, "null argument");
+    *ms = e->last_run_ms;
+    return 0;
+}
+
+int pxd_enable_walk_timing(pxd_engine* e, int enable) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    e->walk_timing = enable != 0;
+    e->walk_launches = 0;
+    e->walk_ms = 0.0;
+    e->walk_bytes = 0.0;
+    return 0;
+}
+
+int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    if (launches) *launches = e->walk_launches;
+    if (total_ms) *total_ms = e->walk_ms;
+    if (alg_bytes) *alg_bytes = e->walk_bytes;
+    return 0;
+}
+
+}  // extern "C"

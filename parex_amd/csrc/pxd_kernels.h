@@ -1,0 +1,935 @@
+// HIP kernels of the expander-decomposition engine (gfx950, wave64).
+//
+// State layout in HBM (one engine may hold a block-diagonal batch of graphs):
+//   per node u      row[u] (u32 arc offset), deg[u], dint[u] (valid-prefix length),
+//                   nbr[row[u] .. row[u]+dint[u]) = the still-valid arcs of u, kept
+//                   as an order-preserving prefix of the original row;
+//                   x[2][u] (FP64 walk double buffer), y[u] = x[u]/dint[u],
+//                   xkey[u] (ordered float of the last walk value), label[u]
+//   per position k  pnix[k] (node), pkey[k] (sort key), pclu[k] (cluster id);
+//                   clusters are contiguous position ranges [cs[c], cs[c+1])
+//   per cluster c   cs, cgraph, cvol, cmin (min node index), clabel, cflag, ...
+//   node list       alist = active nodes in ascending index order (walk and
+//                   cut-contribution passes stream rows in memory order)
+#pragma once
+
+#include "pxd_device.h"
+#include "parex_amd/pxd.h"
+
+namespace pxd {
+
+constexpr int BS = 256;                 // default block size (4 waves)
+constexpr int TILE = 2048;              // LDS sort tile: 256 threads x 8 items
+constexpr int TILE_ITEMS = 8;
+constexpr int SCAN_ITEMS = 4;           // scans: 1024 elements per block
+
+constexpr uint32_t CF_CONN = 1u;        // cluster known to be connected
+constexpr uint32_t CF_CUT = 2u;         // sweep cut applies this iteration
+constexpr uint32_t CF_CC = 4u;          // needs connected components this iteration
+
+struct Counters {
+    uint32_t err;
+    uint32_t nwave, nblock, nlarge;
+    uint32_t ncut, ncc;
+    uint32_t cclist_n;
+    uint32_t hook_changed, jump_changed;
+    uint32_t pad[7];
+};
+
+// ------------------------------------------------------------------ init
+
+__device__ float init_value(uint32_t idx, uint32_t deg, uint32_t seed) {
+    // cuda/devRandomWalk.h:22-47: hash -> minstd_rand -> normal_distribution<float>
+    uint32_t x = seed ^ idx;
+    x = ((x >> 16) ^ x) * 0x45d9f3bu;
+    x = ((x >> 16) ^ x) * 0x45d9f3bu;
+    x = (x >> 16) ^ x;
+    const uint64_t M = 2147483647u;
+    uint64_t s = x % M;
+    if (s == 0) s = 1;
+    s = (s * 48271u) % M;
+    const uint32_t range = 2147483645u;
+    const float S1 = (float)(1.0 / (double)range);
+    const float S2 = S1 / 2;
+    float S3 = (float)(-1.4142135623730950488016887242097);
+    uint32_t u = (uint32_t)s - 1u;
+    if (u > range / 2) { u = range - u; S3 = -S3; }
+    float p = fmaf((float)u, S1, S2);
+    float e = erfcinvf(2.0f * p);
+    float sigma = (float)sqrt((double)deg);
+    return (sigma * S3) * e;
+}
+
+__global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict__ gnode,
+                             const uint32_t* __restrict__ row, uint32_t seed,
+                             uint32_t* __restrict__ deg, uint32_t* __restrict__ dint,
+                             double* __restrict__ x, double* __restrict__ y,
+                             uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
+                             uint8_t* __restrict__ alive, uint32_t* __restrict__ stamp,
+                             uint32_t* __restrict__ pnix, uint32_t* __restrict__ alist,
+                             uint32_t* __restrict__ pclu, const uint32_t* __restrict__ gclu) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = G;                  // graph of u: last g with gnode[g] <= u
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (gnode[mid] <= u) lo = mid; else hi = mid;
+        }
+        uint32_t d = row[u + 1] - row[u];
+        deg[u] = d;
+        dint[u] = d;
+        double v = (double)init_value(u - gnode[lo], d, seed);
+        x[u] = v;
+        y[u] = d ? v / (double)d : 0.0;
+        xkey[u] = ord_f32((float)v);
+        label[u] = 0;
+        alive[u] = 1;
+        stamp[u] = 0;
+        pnix[u] = u;
+        alist[u] = u;
+        pclu[u] = gclu[lo];
+    }
+}
+
+// ------------------------------------------------------------------ walk (K1)
+
+// Lazy walk step for every active node, GS lanes per node, nodes visited in
+// index order: x'[u] = 0.9 * sum_{v in valid(u)} x[v]/dInt[v] + 0.1 * x[u]
+// (cuda/devRandomWalk.h:117-203).  The neighbour sum is exact (fixed point),
+// so its bits do not depend on GS or on the lane that adds a term.
+template <int GS>
+__global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __restrict__ alist,
+                                             const uint32_t* __restrict__ row,
+                                             const uint32_t* __restrict__ dint,
+                                             const uint32_t* __restrict__ nbr,
+                                             const double* __restrict__ y,
+                                             const double* __restrict__ xc, double* __restrict__ xn,
+                                             double* __restrict__ yn, uint32_t* __restrict__ xkey,
+                                             Counters* cnt) {
+    const uint32_t lane = threadIdx.x & (GS - 1);
+    const uint32_t groups = gridDim.x * (BS / GS);
+    uint32_t ovf = 0;
+    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < na; i += groups) {
+        const uint32_t u = alist[i];
+        const uint32_t start = row[u], d = dint[u];
+        Fix128 acc = {0ull, 0ull};
+        for (uint32_t j = lane; j < d; j += GS) fix_add_term(acc, y[nbr[start + j]], ovf);
+        acc = group_sum_fix<GS>(acc);
+        if (lane == 0) {
+            if (d > (1u << 24)) ovf = 1;
+            const double s = fix_to_f64(acc);
+            const double v = s * (1.0 - RW_STAY) + xc[u] * RW_STAY;
+            xn[u] = v;
+            yn[u] = d ? v / (double)d : 0.0;
+            xkey[u] = ord_f32((float)v);
+        }
+    }
+    if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
+}
+
+// ------------------------------------------------------------------ cut contribution (K4)
+
+// ediff[u] = sum over valid arcs of (neighbour sorts before u ? -1 : +1), where
+// "before" compares walk keys and breaks ties by node index
+// (cuda/devSweepCut.h:20-112).
+template <int GS>
+__global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __restrict__ alist,
+                                                const uint32_t* __restrict__ row,
+                                                const uint32_t* __restrict__ dint,
+                                                const uint32_t* __restrict__ nbr,
+                                                const uint32_t* __restrict__ xkey,
+                                                int32_t* __restrict__ ediff) {
+    const uint32_t lane = threadIdx.x & (GS - 1);
+    const uint32_t groups = gridDim.x * (BS / GS);
+    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < na; i += groups) {
+        const uint32_t u = alist[i];
+        const uint32_t start = row[u], d = dint[u];
+        const uint32_t mk = xkey[u];
+        int32_t c = 0;
+        for (uint32_t j = lane; j < d; j += GS) {
+            const uint32_t v = nbr[start + j];
+            const uint32_t vk = xkey[v];
+            const bool before = (vk != mk) ? (vk < mk) : (v < u);
+            c += before ? -1 : 1;
+        }
+        c = group_sum<GS>(c);
+        if (lane == 0) ediff[u] = c;
+    }
+}
+
+// ------------------------------------------------------------------ generic scan
+
+// out[i] = sum_{j<i} in[j] for i < n, out[n] = total (3-level reduce-then-scan).
+__global__ void __launch_bounds__(BS) k_scan_up(uint32_t n, const uint32_t* __restrict__ in,
+                                                uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t lds[BS / WAVE];
+    const uint32_t base = blockIdx.x * (BS * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) s += (base + i < n) ? in[base + i] : 0u;
+    uint32_t tot;
+    block_excl_scan<BS>(s, lds, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __restrict__ in,
+                                                  uint32_t* __restrict__ out,
+                                                  const uint32_t* __restrict__ boff) {
+    __shared__ uint32_t lds[BS / WAVE];
+    const uint32_t base = blockIdx.x * (BS * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) { v[i] = (base + i < n) ? in[base + i] : 0u; s += v[i]; }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<BS>(s, lds, tot) + (boff ? boff[blockIdx.x] : 0u);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == BS - 1) out[n] = run;
+}
+
+// ------------------------------------------------------------------ segmented sort
+
+// Sort classes: size <= 1 nothing, <= 64 one wave, <= TILE one block in LDS,
+// larger: LSD radix over 8-bit digits with per-tile histograms.
+__global__ void k_classify(uint32_t nseg, const uint32_t* __restrict__ segs,
+                           uint32_t* __restrict__ lwave, uint32_t* __restrict__ lblock,
+                           uint32_t* __restrict__ llarge, Counters* cnt) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nseg; c += gridDim.x * blockDim.x) {
+        const uint32_t s = segs[c + 1] - segs[c];
+        if (s <= 1) continue;
+        if (s <= WAVE) lwave[atomicAdd(&cnt->nwave, 1u)] = c;
+        else if (s <= (uint32_t)TILE) lblock[atomicAdd(&cnt->nblock, 1u)] = c;
+        else llarge[atomicAdd(&cnt->nlarge, 1u)] = c;
+    }
+}
+
+// one wave per segment (<= 64 items): rank by counting, stable on input order
+__global__ void __launch_bounds__(BS) k_sort_wave(uint32_t nlist, const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ segs,
+                                                  uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    const uint32_t w = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE;
+    if (w >= nlist) return;
+    const uint32_t lane = lane_id();
+    const uint32_t c = list[w];
+    const uint32_t s0 = segs[c], s = segs[c + 1] - s0;
+    const bool valid = lane < s;
+    const uint32_t k = valid ? key[s0 + lane] : 0xFFFFFFFFu;
+    const uint32_t v = valid ? val[s0 + lane] : 0u;
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < s; ++j) {
+        const uint32_t kj = __shfl(k, (int)j);
+        rank += (kj < k || (kj == k && j < lane)) ? 1u : 0u;
+    }
+    if (valid) {
+        key[s0 + rank] = k;
+        val[s0 + rank] = v;
+    }
+}
+
+// Stable LDS split sort of cnt (<= TILE) packed (key << 32 | val) items on the
+// key bits set in `bits`; 256 threads, thread t owns items 8t..8t+7.
+// Returns the buffer holding the result.
+__device__ uint64_t* lds_split_sort(uint64_t* b0, uint64_t* b1, uint32_t cnt, uint32_t bits,
+                                    uint32_t* scratch) {
+    const uint32_t t = threadIdx.x;
+    for (int b = 0; b < 32; ++b) {
+        if (!((bits >> b) & 1u)) continue;
+        uint64_t e[TILE_ITEMS];
+        uint32_t z = 0;
+#pragma unroll
+        for (int i = 0; i < TILE_ITEMS; ++i) {
+            const uint32_t p = t * TILE_ITEMS + i;
+            e[i] = p < cnt ? b0[p] : 0ull;
+            if (p < cnt && !((e[i] >> (32 + b)) & 1ull)) ++z;
+        }
+        uint32_t Z;
+        uint32_t run = block_excl_scan<BS>(z, scratch, Z);
+#pragma unroll
+        for (int i = 0; i < TILE_ITEMS; ++i) {
+            const uint32_t p = t * TILE_ITEMS + i;
+            if (p < cnt) {
+                const bool one = (e[i] >> (32 + b)) & 1ull;
+                const uint32_t dst = one ? (Z + p - run) : run;
+                if (!one) ++run;
+                b1[dst] = e[i];
+            }
+        }
+        __syncthreads();
+        uint64_t* tmp = b0; b0 = b1; b1 = tmp;
+    }
+    return b0;
+}
+
+// one block per segment of 65..TILE items, sorted in place
+__global__ void __launch_bounds__(BS) k_sort_block(uint32_t nlist, const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ segs,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    __shared__ uint64_t buf0[TILE], buf1[TILE];
+    __shared__ uint32_t scratch[BS / WAVE];
+    __shared__ uint32_t s_or, s_and;
+    if (blockIdx.x >= nlist) return;
+    const uint32_t c = list[blockIdx.x];
+    const uint32_t s0 = segs[c], cnt = segs[c + 1] - s0;
+    if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
+    __syncthreads();
+    uint32_t lor = 0u, land = 0xFFFFFFFFu;
+    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
+        const uint32_t k = key[s0 + p];
+        buf0[p] = ((uint64_t)k << 32) | val[s0 + p];
+        lor |= k;
+        land &= k;
+    }
+    atomicOr(&s_or, lor);
+    atomicAnd(&s_and, land);
+    __syncthreads();
+    const uint32_t bits = s_or ^ s_and;
+    uint64_t* r = lds_split_sort(buf0, buf1, cnt, bits, scratch);
+    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
+        key[s0 + p] = (uint32_t)(r[p] >> 32);
+        val[s0 + p] = (uint32_t)r[p];
+    }
+}
+
+// large segments: tile descriptors (segment, tile index) are built on the host
+struct LargeSeg {
+    uint32_t start, len, ntiles, hbase;
+};
+
+__global__ void __launch_bounds__(BS) k_tile_hist(uint32_t ntiles, const uint32_t* __restrict__ tseg,
+                                                  const uint32_t* __restrict__ tidx,
+                                                  const LargeSeg* __restrict__ ls,
+                                                  const uint32_t* __restrict__ key, int shift,
+                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    if (blockIdx.x >= ntiles) return;
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const LargeSeg L = ls[tseg[blockIdx.x]];
+    const uint32_t t = tidx[blockIdx.x];
+    const uint32_t b = L.start + t * TILE;
+    const uint32_t e = min(L.start + L.len, b + (uint32_t)TILE);
+    for (uint32_t p = b + threadIdx.x; p < e; p += BS) atomicAdd(&h[(key[p] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[L.hbase + threadIdx.x * L.ntiles + t] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint32_t* __restrict__ tseg,
+                                                     const uint32_t* __restrict__ tidx,
+                                                     const LargeSeg* __restrict__ ls,
+                                                     const uint32_t* __restrict__ hscan,
+                                                     const uint32_t* __restrict__ ksrc,
+                                                     const uint32_t* __restrict__ vsrc, int shift,
+                                                     uint32_t* __restrict__ kdst, uint32_t* __restrict__ vdst) {
+    __shared__ uint64_t buf0[TILE], buf1[TILE];
+    __shared__ uint32_t h[256], hs[256], goff[256];
+    __shared__ uint32_t scratch[BS / WAVE];
+    __shared__ uint32_t s_or, s_and;
+    if (blockIdx.x >= ntiles) return;
+    const LargeSeg L = ls[tseg[blockIdx.x]];
+    const uint32_t t = tidx[blockIdx.x];
+    const uint32_t b = L.start + t * TILE;
+    const uint32_t cnt = min(L.start + L.len, b + (uint32_t)TILE) - b;
+    h[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
+    __syncthreads();
+    uint32_t lor = 0u, land = 0xFFFFFFFFu;
+    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
+        const uint32_t k = ksrc[b + p];
+        buf0[p] = ((uint64_t)k << 32) | vsrc[b + p];
+        const uint32_t d = (k >> shift) & 255u;
+        atomicAdd(&h[d], 1u);
+        lor |= d;
+        land &= d;
+    }
+    atomicOr(&s_or, lor);
+    atomicAnd(&s_and, land);
+    __syncthreads();
+    {
+        uint32_t tot;
+        hs[threadIdx.x] = block_excl_scan<BS>(h[threadIdx.x], scratch, tot);
+        const uint32_t gi = L.hbase + threadIdx.x * L.ntiles + t;
+        goff[threadIdx.x] = hscan[gi] - hscan[L.hbase] + L.start;
+    }
+    __syncthreads();
+    const uint32_t bits = ((s_or ^ s_and) & 255u) << shift;
+    uint64_t* r = lds_split_sort(buf0, buf1, cnt, bits, scratch);
+    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
+        const uint64_t e = r[p];
+        const uint32_t d = (uint32_t)(e >> (32 + shift)) & 255u;
+        const uint32_t dst = goff[d] + (p - hs[d]);
+        kdst[dst] = (uint32_t)(e >> 32);
+        vdst[dst] = (uint32_t)e;
+    }
+}
+
+// ------------------------------------------------------------------ sweep scan
+
+// Phase A: per block of 1024 positions, the (edgeDiff, volume) total of the
+// block's last segment and whether the block contains a segment start.
+__global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __restrict__ pnix,
+                                                const uint32_t* __restrict__ pclu,
+                                                const uint32_t* __restrict__ cs,
+                                                const int32_t* __restrict__ ediff,
+                                                const uint32_t* __restrict__ dint,
+                                                uint32_t* __restrict__ tail_ed, uint32_t* __restrict__ tail_vol,
+                                                uint32_t* __restrict__ has_start) {
+    __shared__ uint32_t s_ed, s_vol, s_start;
+    const uint32_t b0 = blockIdx.x * (BS * SCAN_ITEMS);
+    const uint32_t b1 = min(na, b0 + BS * SCAN_ITEMS);
+    if (threadIdx.x == 0) { s_ed = 0u; s_vol = 0u; s_start = 0u; }
+    __syncthreads();
+    const uint32_t lastc = pclu[b1 - 1];
+    uint32_t ed = 0u, vol = 0u, st = 0u;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint32_t k = b0 + threadIdx.x * SCAN_ITEMS + i;
+        if (k >= b1) break;
+        const uint32_t c = pclu[k];
+        if (cs[c] == k) st = 1u;
+        if (c == lastc) {
+            const uint32_t u = pnix[k];
+            ed += (uint32_t)ediff[u];
+            vol += dint[u];
+        }
+    }
+    ed = group_sum<WAVE>(ed);
+    vol = group_sum<WAVE>(vol);
+    st = group_or<WAVE>(st);
+    if (lane_id() == 0) {
+        atomicAdd(&s_ed, ed);
+        atomicAdd(&s_vol, vol);
+        atomicOr(&s_start, st);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        tail_ed[blockIdx.x] = s_ed;
+        tail_vol[blockIdx.x] = s_vol;
+        has_start[blockIdx.x] = s_start;
+    }
+}
+
+// Phase B: carry into every block = inclusive segmented sum of the previous
+// blocks' tails (restart after a block with a start).  One block of 1024.
+__global__ void __launch_bounds__(1024) k_sweep_b(uint32_t nb, const uint32_t* __restrict__ tail_ed,
+                                                  const uint32_t* __restrict__ tail_vol,
+                                                  const uint32_t* __restrict__ has_start,
+                                                  uint32_t* __restrict__ carry_ed,
+                                                  uint32_t* __restrict__ carry_vol) {
+    __shared__ uint32_t f[1024], e[1024], v[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = t * per, hi = min(nb, lo + per);
+    uint32_t lf = 0u, le = 0u, lv = 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        if (has_start[i]) { lf = 1u; le = tail_ed[i]; lv = tail_vol[i]; }
+        else { le += tail_ed[i]; lv += tail_vol[i]; }
+    }
+    f[t] = lf; e[t] = le; v[t] = lv;
+    __syncthreads();
+    // inclusive segmented Hillis-Steele scan over the 1024 chunk aggregates
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t pf = 0u, pe = 0u, pv = 0u;
+        const bool take = t >= off;
+        if (take) { pf = f[t - off]; pe = e[t - off]; pv = v[t - off]; }
+        __syncthreads();
+        if (take && !f[t]) { e[t] += pe; v[t] += pv; f[t] = pf; }
+        __syncthreads();
+    }
+    uint32_t ce = t ? e[t - 1] : 0u, cv = t ? v[t - 1] : 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        carry_ed[i] = ce;
+        carry_vol[i] = cv;
+        if (has_start[i]) { ce = tail_ed[i]; cv = tail_vol[i]; }
+        else { ce += tail_ed[i]; cv += tail_vol[i]; }
+    }
+}
+
+// Phase C: inclusive segmented prefix of (edgeDiff, volume, 1) per position,
+// conductance (cuda/devSweepCut.h:403-432) and per-cluster rightmost argmin
+// packed as (ordered sparsity << 32) | ~offset for atomicMin.
+__global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __restrict__ pnix,
+                                                const uint32_t* __restrict__ pclu,
+                                                const uint32_t* __restrict__ cs,
+                                                const uint32_t* __restrict__ cvol,
+                                                const int32_t* __restrict__ ediff,
+                                                const uint32_t* __restrict__ dint,
+                                                const uint32_t* __restrict__ carry_ed,
+                                                const uint32_t* __restrict__ carry_vol,
+                                                unsigned long long* __restrict__ cbest,
+                                                pxd_node_data* __restrict__ nd) {
+    __shared__ uint32_t f[BS], e[BS], v[BS];
+    const uint32_t b0 = blockIdx.x * (BS * SCAN_ITEMS);
+    const uint32_t t = threadIdx.x;
+    uint32_t ed[SCAN_ITEMS], vo[SCAN_ITEMS], cc[SCAN_ITEMS], uu[SCAN_ITEMS];
+    bool st[SCAN_ITEMS], ok[SCAN_ITEMS];
+    uint32_t lf = 0u, le = 0u, lv = 0u;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint32_t k = b0 + t * SCAN_ITEMS + i;
+        ok[i] = k < na;
+        if (ok[i]) {
+            cc[i] = pclu[k];
+            uu[i] = pnix[k];
+            st[i] = cs[cc[i]] == k;
+            ed[i] = (uint32_t)ediff[uu[i]];
+            vo[i] = dint[uu[i]];
+        } else {
+            cc[i] = 0u; uu[i] = 0u; st[i] = true; ed[i] = 0u; vo[i] = 0u;
+        }
+        if (st[i]) { lf = 1u; le = ed[i]; lv = vo[i]; }
+        else { le += ed[i]; lv += vo[i]; }
+    }
+    f[t] = lf; e[t] = le; v[t] = lv;
+    __syncthreads();
+    for (uint32_t off = 1; off < BS; off <<= 1) {
+        uint32_t pf = 0u, pe = 0u, pv = 0u;
+        const bool take = t >= off;
+        if (take) { pf = f[t - off]; pe = e[t - off]; pv = v[t - off]; }
+        __syncthreads();
+        if (take && !f[t]) { e[t] += pe; v[t] += pv; f[t] = pf; }
+        __syncthreads();
+    }
+    // exclusive prefix for this thread's first item, carrying across blocks
+    uint32_t re, rv;
+    bool open;                         // prefix still continues a segment from before the block
+    if (t == 0) { re = 0u; rv = 0u; open = true; }
+    else { re = e[t - 1]; rv = v[t - 1]; open = !f[t - 1]; }
+    if (open) { re += carry_ed[blockIdx.x]; rv += carry_vol[blockIdx.x]; }
+    uint32_t runc = 0xFFFFFFFFu;
+    unsigned long long runmin = ~0ull;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (st[i]) { re = ed[i]; rv = vo[i]; }
+        else { re += ed[i]; rv += vo[i]; }
+        if (!ok[i]) continue;
+        const uint32_t k = b0 + t * SCAN_ITEMS + i;
+        const uint32_t c = cc[i];
+        const uint32_t off = k - cs[c] + 1u;
+        const uint32_t tot = cvol[c];
+        const uint32_t denom = (rv >= tot) ? 0u : ((rv < tot - rv) ? rv : (tot - rv));
+        const float sp = denom > 0u ? ((float)re / (float)denom) : 2.0f;
+        const unsigned long long pk = ((unsigned long long)ord_f32(sp) << 32) | (0xFFFFFFFFu - off);
+        if (c != runc) {
+            if (runc != 0xFFFFFFFFu) atomicMin(&cbest[runc], runmin);
+            runc = c;
+            runmin = pk;
+        } else if (pk < runmin) {
+            runmin = pk;
+        }
+        if (nd) {
+            pxd_node_data o;
+            o.nix = uu[i]; o.prefixVolume = rv; o.prefixEdgeDiff = (int32_t)re; o.offsetInCluster = off;
+            nd[k] = o;
+        }
+    }
+    // one atomic per wave when the whole wave ends in the same cluster
+    const uint32_t c0 = __shfl(runc, 0);
+    if (__all(runc == c0) && c0 != 0xFFFFFFFFu) {
+        for (int offs = WAVE / 2; offs > 0; offs >>= 1) {
+            const unsigned long long o = __shfl_xor(runmin, offs);
+            runmin = o < runmin ? o : runmin;
+        }
+        if (lane_id() == 0) atomicMin(&cbest[c0], runmin);
+    } else if (runc != 0xFFFFFFFFu) {
+        atomicMin(&cbest[runc], runmin);
+    }
+}
+
+
+// ------------------------------------------------------------------ cut decision
+
+// cuda/devSweepCut.h:701-709 (threshold filter) for every cluster; flags the
+// clusters that need connected components this iteration.
+__global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbest, double thr,
+                         uint32_t* __restrict__ cflag, uint32_t* __restrict__ coff,
+                         float* __restrict__ csp, Counters* cnt) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const unsigned long long b = cbest[c];
+        const float sp = unord_f32((uint32_t)(b >> 32));
+        const uint32_t off = 0xFFFFFFFFu - (uint32_t)b;
+        csp[c] = sp;
+        coff[c] = off;
+        uint32_t f = cflag[c] & CF_CONN;
+        if ((double)sp < thr) { f |= CF_CUT; atomicAdd(&cnt->ncut, 1u); }
+        if ((f & CF_CUT) || !(f & CF_CONN)) { f |= CF_CC; atomicAdd(&cnt->ncc, 1u); }
+        cflag[c] = f;
+    }
+}
+
+// cuda/devPartition.h:545-622: positions beyond the cut offset form the second
+// part.  Members of CC clusters get a part id, a fresh root and join the CC list.
+__global__ void k_mark(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                       const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
+                       const uint32_t* __restrict__ coff, uint32_t iter, uint32_t* __restrict__ part,
+                       uint32_t* __restrict__ root, uint32_t* __restrict__ stamp,
+                       uint32_t* __restrict__ cclist, Counters* cnt) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pclu[k];
+        const uint32_t f = cflag[c];
+        if (!(f & CF_CC)) continue;
+        const uint32_t u = pnix[k];
+        const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
+        part[u] = (c << 1) | side;
+        root[u] = u;
+        stamp[u] = iter;
+        cclist[atomicAdd(&cnt->cclist_n, 1u)] = u;
+    }
+}
+
+// Hooking round: for every same-part valid arc with different parent pointers,
+// atomicMin the larger pointer's slot to the smaller (parents only decrease,
+// so the root of a tree is its minimum node).  Any difference seen schedules
+// another round; the loop ends only when all arcs agree on flat trees.
+template <int GS>
+__global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ row,
+                                                const uint32_t* __restrict__ dint,
+                                                const uint32_t* __restrict__ nbr,
+                                                const uint32_t* __restrict__ part,
+                                                uint32_t* __restrict__ root, Counters* cnt) {
+    const uint32_t lane = threadIdx.x & (GS - 1);
+    const uint32_t groups = gridDim.x * (BS / GS);
+    uint32_t changed = 0;
+    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
+        const uint32_t u = list[i];
+        const uint32_t start = row[u], d = dint[u];
+        const uint32_t pp = part[u];
+        const uint32_t pu = root[u];
+        for (uint32_t j = lane; j < d; j += GS) {
+            const uint32_t v = nbr[start + j];
+            if (part[v] != pp) continue;
+            const uint32_t pv = root[v];
+            if (pv != pu) {
+                atomicMin(&root[max(pu, pv)], min(pu, pv));
+                changed = 1;
+            }
+        }
+    }
+    if (changed) atomicOr(&cnt->hook_changed, 1u);
+}
+
+// pointer jumping: root[u] <- root[root[u]]
+__global__ void k_cc_jump(uint32_t nl, const uint32_t* __restrict__ list, uint32_t* __restrict__ root,
+                          Counters* cnt) {
+    uint32_t changed = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        const uint32_t r = root[u];
+        const uint32_t rr = root[r];
+        if (rr != r) { root[u] = rr; changed = 1; }
+    }
+    if (changed) atomicOr(&cnt->jump_changed, 1u);
+}
+
+// regroup key: component root for CC members, the cluster's minimum node otherwise
+__global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                      const uint32_t* __restrict__ cflag, const uint32_t* __restrict__ cmin,
+                      const uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pclu[k];
+        key[k] = (cflag[c] & CF_CC) ? root[pnix[k]] : cmin[c];
+    }
+}
+
+__global__ void k_run_flags(uint32_t na, const uint32_t* __restrict__ key, uint32_t* __restrict__ f) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
+        f[k] = (k == 0 || key[k] != key[k - 1]) ? 1u : 0u;
+}
+
+// new cluster table from runs of equal regroup keys (sorted by minimum node)
+__global__ void k_build_clusters(uint32_t na, uint32_t G, const uint32_t* __restrict__ gnode,
+                                 const uint32_t* __restrict__ key, const uint32_t* __restrict__ f,
+                                 const uint32_t* __restrict__ fs, uint32_t* __restrict__ cs,
+                                 uint32_t* __restrict__ cmin, uint32_t* __restrict__ cgraph,
+                                 uint32_t* __restrict__ cflag, uint32_t* __restrict__ pclu) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = fs[k] + f[k] - 1u;
+        pclu[k] = c;
+        if (f[k]) {
+            const uint32_t r = key[k];
+            uint32_t lo = 0, hi = G;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (gnode[mid] <= r) lo = mid; else hi = mid;
+            }
+            cs[c] = k;
+            cmin[c] = r;
+            cgraph[c] = lo;
+            cflag[c] = CF_CONN;
+        }
+        if (k == na - 1) cs[c + 1] = na;
+    }
+}
+
+__global__ void k_graph_first(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gfirst) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        if (c == 0 || cgraph[c] != cgraph[c - 1]) gfirst[cgraph[c]] = c;
+}
+
+// ------------------------------------------------------------------ cluster statistics
+
+__global__ void k_stats_reset(uint32_t nc, uint32_t* __restrict__ cmaxo, uint32_t* __restrict__ cmino,
+                              unsigned long long* __restrict__ csum, unsigned long long* __restrict__ cbest) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        if (cmaxo) { cmaxo[c] = 0u; cmino[c] = 0xFFFFFFFFu; csum[c] = 0ull; }
+        if (cbest) cbest[c] = ~0ull;
+    }
+}
+
+// max / min of (float)x over each cluster as ordered bits (-0 < +0)
+__global__ void __launch_bounds__(BS) k_stats_minmax(uint32_t na, const uint32_t* __restrict__ pnix,
+                                                     const uint32_t* __restrict__ pclu,
+                                                     const double* __restrict__ x,
+                                                     uint32_t* __restrict__ cmaxo, uint32_t* __restrict__ cmino) {
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool ok = k < na;
+        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+        const uint32_t o = ok ? ord_f32((float)x[pnix[k]]) : 0u;
+        const uint32_t c0 = __shfl(c, 0);
+        if (c0 == 0xFFFFFFFFu) {
+            continue;                      // whole wave past the end: nothing to add
+        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
+            uint32_t mx = ok ? o : 0u, mn = ok ? o : 0xFFFFFFFFu;
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                mx = max(mx, __shfl_xor(mx, off));
+                mn = min(mn, __shfl_xor(mn, off));
+            }
+            if (lane_id() == 0) { atomicMax(&cmaxo[c0], mx); atomicMin(&cmino[c0], mn); }
+        } else if (ok) {
+            atomicMax(&cmaxo[c], o);
+            atomicMin(&cmino[c], o);
+        }
+    }
+}
+
+__device__ __forceinline__ int ceil_log2_u32(uint32_t k) {
+    return k <= 1u ? 0 : 32 - __clz((int)(k - 1u));
+}
+
+// fixed-point scale K per cluster: amax < 2^E, K = 62 - E - ceil(log2(count))
+__global__ void k_stats_scale(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cmaxo,
+                              const uint32_t* __restrict__ cmino, int32_t* __restrict__ cK) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const float amax = fmaxf(fabsf(unord_f32(cmaxo[c])), fabsf(unord_f32(cmino[c])));
+        int E = 0;
+        if (amax != 0.0f) frexpf(amax, &E);
+        cK[c] = 62 - E - ceil_log2_u32(cs[c + 1] - cs[c]);
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* __restrict__ pnix,
+                                                  const uint32_t* __restrict__ pclu,
+                                                  const double* __restrict__ x, const int32_t* __restrict__ cK,
+                                                  unsigned long long* __restrict__ csum) {
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool ok = k < na;
+        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+        int64_t term = 0;
+        if (ok) term = (int64_t)ldexp((double)(float)x[pnix[k]], cK[c]);
+        const uint32_t c0 = __shfl(c, 0);
+        if (c0 == 0xFFFFFFFFu) {
+            continue;                      // whole wave past the end: nothing to add
+        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
+            for (int off = WAVE / 2; off > 0; off >>= 1) term += __shfl_xor(term, off);
+            if (lane_id() == 0) atomicAdd(&csum[c0], (unsigned long long)term);
+        } else if (ok) {
+            atomicAdd(&csum[c], (unsigned long long)term);
+        }
+    }
+}
+
+// cuda/devPartition.h:173-184, 624-650: keep flag, average, FP32 sum
+__global__ void k_stats_final(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cmaxo,
+                              const uint32_t* __restrict__ cmino, const int32_t* __restrict__ cK,
+                              const unsigned long long* __restrict__ csum, float rw_thr,
+                              uint32_t* __restrict__ ckeep, double* __restrict__ cavg) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t count = cs[c + 1] - cs[c];
+        const float rw_sum = (float)ldexp(i64_to_f64((int64_t)csum[c]), -cK[c]);
+        const float pot = unord_f32(cmaxo[c]) - unord_f32(cmino[c]);
+        ckeep[c] = (pot >= rw_thr && count >= 2u) ? 1u : 0u;
+        cavg[c] = (double)rw_sum / (double)count;
+    }
+}
+
+// labels: survivors dense per graph in cluster order, deactivated
+// smallest - index - 1 (cuda/devPartition.h:686-698)
+__global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const uint32_t* __restrict__ gfirst,
+                         const uint32_t* __restrict__ ckeep, const uint32_t* __restrict__ kscan,
+                         const int32_t* __restrict__ gsmall, int32_t* __restrict__ gsmall_next,
+                         uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t g = cgraph[c];
+        const uint32_t f = gfirst[g];
+        if (ckeep[c]) {
+            clabel[c] = (int32_t)(kscan[c] - kscan[f]);
+            atomicAdd(&gsurv[g], 1u);
+        } else {
+            const int32_t lab = gsmall[g] - (int32_t)(c - f) - 1;
+            clabel[c] = lab;
+            atomicMin(&gsmall_next[g], lab);
+        }
+    }
+}
+
+__global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive, int32_t* __restrict__ gsmall,
+                               int32_t* __restrict__ gsmall_next, uint32_t* __restrict__ gsurv,
+                               int32_t* __restrict__ giter, uint32_t iter1) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        if (gactive[g]) {
+            gsmall[g] = gsmall_next[g];
+            if (gsurv[g] == 0u && giter[g] == 0) giter[g] = (int32_t)iter1 + 1;
+        }
+        gsmall_next[g] = 0;
+        gsurv[g] = 0u;
+    }
+}
+
+// per position: write labels, recentre survivors (x -= avg), refresh y
+__global__ void k_apply(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                        const uint32_t* __restrict__ ckeep, const int32_t* __restrict__ clabel,
+                        const double* __restrict__ cavg, const uint32_t* __restrict__ dint,
+                        int32_t* __restrict__ label, double* __restrict__ x, double* __restrict__ y,
+                        uint8_t* __restrict__ alive) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pclu[k], u = pnix[k];
+        label[u] = clabel[c];
+        if (ckeep[c]) {
+            const double v = x[u] - cavg[c];
+            x[u] = v;
+            const uint32_t d = dint[u];
+            y[u] = d ? v / (double)d : 0.0;
+        } else {
+            alive[u] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ compaction
+
+__global__ void k_pos_keepflags(uint32_t na, const uint32_t* __restrict__ pclu, const uint32_t* __restrict__ ckeep,
+                                uint32_t* __restrict__ f) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
+        f[k] = ckeep[pclu[k]];
+}
+
+__global__ void k_pos_compact(uint32_t na, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,
+                              const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                              const uint32_t* __restrict__ cscan, uint32_t* __restrict__ pnix_o,
+                              uint32_t* __restrict__ pclu_o) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
+        if (f[k]) { pnix_o[fs[k]] = pnix[k]; pclu_o[fs[k]] = cscan[pclu[k]]; }
+}
+
+struct ClusterCols {
+    uint32_t *cs, *cgraph, *cmin, *cflag, *cvol;
+    int32_t* clabel;
+};
+
+__global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __restrict__ ckeep,
+                              const uint32_t* __restrict__ cscan, const uint32_t* __restrict__ pscan,
+                              ClusterCols in, ClusterCols out) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        if (!ckeep[c]) continue;
+        const uint32_t d = cscan[c];
+        out.cs[d] = pscan[in.cs[c]];
+        out.cgraph[d] = in.cgraph[c];
+        out.cmin[d] = in.cmin[c];
+        out.cflag[d] = in.cflag[c] & CF_CONN;
+        out.clabel[d] = in.clabel[c];
+        out.cvol[d] = in.cvol[c];
+        if (d + 1 == cscan[nc]) out.cs[d + 1] = na_new;
+    }
+}
+
+__global__ void k_alive_flags(uint32_t n, const uint32_t* __restrict__ alist, const uint8_t* __restrict__ alive,
+                              uint32_t* __restrict__ f) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        f[i] = alive[alist[i]];
+}
+
+__global__ void k_compact_u32(uint32_t n, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,
+                              const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (f[i]) out[fs[i]] = in[i];
+}
+
+// ------------------------------------------------------------------ disable arcs (K23)
+
+// For surviving CC members: keep arcs whose endpoint has the same component
+// root, compacting them order-preservingly to the row front
+// (cuda/devPartition.h:80-146); refresh dInt and y when the row changed.
+template <int GS>
+__global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __restrict__ list,
+                                                const uint8_t* __restrict__ alive,
+                                                const uint32_t* __restrict__ row,
+                                                uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
+                                                const uint32_t* __restrict__ root,
+                                                const double* __restrict__ x, double* __restrict__ y) {
+    const uint32_t lane = threadIdx.x & (GS - 1);
+    const uint32_t groups = gridDim.x * (BS / GS);
+    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
+        const uint32_t u = list[i];
+        if (!alive[u]) continue;                      // uniform within the group
+        const uint32_t start = row[u], d = dint[u];
+        const uint32_t ru = root[u];
+        uint32_t kept = 0;
+        for (uint32_t j0 = 0; j0 < d; j0 += GS) {
+            const uint32_t j = j0 + lane;
+            uint32_t v = 0;
+            bool keep = false;
+            if (j < d) { v = nbr[start + j]; keep = root[v] == ru; }
+            const uint64_t bal = group_ballot<GS>(keep);
+            const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            if (keep && kept + before != j) nbr[start + kept + before] = v;
+            kept += (uint32_t)__popcll(bal);
+        }
+        if (lane == 0 && kept != d) {
+            dint[u] = kept;
+            y[u] = kept ? x[u] / (double)kept : 0.0;
+        }
+    }
+}
+
+__global__ void k_volume(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                         const uint32_t* __restrict__ dint, uint32_t* __restrict__ cvol) {
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool ok = k < na;
+        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+        uint32_t d = ok ? dint[pnix[k]] : 0u;
+        const uint32_t c0 = __shfl(c, 0);
+        if (c0 == 0xFFFFFFFFu) {
+            continue;                      // whole wave past the end: nothing to add
+        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
+            d = group_sum<WAVE>(d);
+            if (lane_id() == 0) atomicAdd(&cvol[c0], d);
+        } else if (ok) {
+            atomicAdd(&cvol[c], d);
+        }
+    }
+}
+
+__global__ void k_gather(uint32_t n, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ src,
+                         uint32_t* __restrict__ dst) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
+__global__ void k_fill_u32(uint32_t n, uint32_t* __restrict__ p, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// count of still-valid arcs over all nodes (getNumCutEdges)
+__global__ void k_sum_u32(uint32_t n, const uint32_t* __restrict__ a, unsigned long long* __restrict__ out) {
+    unsigned long long s = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i];
+    s = group_sum<WAVE>(s);
+    if (lane_id() == 0) atomicAdd(out, s);
+}
+
+}  // namespace pxd

@@ -1,0 +1,148 @@
+"""GPU parity: the HIP engine against the oracle, step by step and end to end.
+
+The oracle is seeded with the device's initial walk vector (the reference's own
+lockstep pattern, tests/cudaTest.cpp:50-69); from there every quantity must be
+bit-identical: FP64 walk state, sort order, NodeData prefixes, sweep cuts,
+labels, internal degrees, iteration counts and the final partition.
+"""
+import numpy as np
+import pytest
+
+import parex_amd as px
+from oracle_ctypes import Oracle, init_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def graph_cases():
+    return {
+        "sbm_c1": lambda: px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582),
+        "torus32": lambda: px.torus(32, 10, 1),
+        "rmat10": lambda: px.rmat(10, 8, 0.57, 0.19, 0.19, 7),
+        "path300": lambda: px.from_edges(300, np.arange(299), np.arange(1, 300)),
+        "islands": lambda: px.from_edges(40, [0, 1, 2, 5, 6, 7, 20, 21, 22, 22],
+                                         [1, 2, 0, 6, 7, 5, 21, 22, 23, 24]),
+        "sbm_small_seed9": lambda: px.sbm(600, 5, 0.08, 0.004, 9),
+    }
+
+
+def _engine(g, **kw):
+    dm = px.DeviceManager(px.Params(**kw))
+    dm.initialize(g)
+    return dm
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def test_init_vector_matches_oracle_within_ulps():
+    g = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
+    dm = _engine(g)
+    dev = dm.readRandomWalkValuesF64().astype(np.float32)
+    ref = init_vector(g.degree(), 6582).astype(np.float32)
+    # HIP erfcinvf vs the oracle's FP64 Newton erfcinv rounded to float
+    ulp = np.abs(dev.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 4, f"max ulp distance {ulp.max()}"
+    # moments: N(0, sqrt(deg)) -> x/sqrt(deg) ~ N(0,1)
+    z = dev[g.degree() > 0] / np.sqrt(g.degree()[g.degree() > 0])
+    assert abs(z.mean()) < 0.1 and abs(z.std() - 1.0) < 0.1
+
+
+@pytest.mark.parametrize("name", list(graph_cases()))
+def test_walk_lockstep(name):
+    g = graph_cases()[name]()
+    dm = _engine(g)
+    x0 = dm.readRandomWalkValuesF64()
+    ox = Oracle(g, init_values=x0)
+    for step in range(16):
+        dm.iterateRandomWalk()
+        ox.walk()
+        a, b = dm.readRandomWalkValuesF64(), ox.values()
+        bad = np.nonzero(_bits(a) != _bits(b))[0]
+        assert bad.size == 0, f"{name} step {step}: {bad.size} values differ, first {bad[:5]} {a[bad[:3]]} {b[bad[:3]]}"
+
+
+def _compare_sweep(dm, ox, tag):
+    na, nc, ncut = dm.counts()
+    ona, onc, oncut = ox.counts()
+    assert (na, nc, ncut) == (ona, onc, oncut), f"{tag}: counts {na, nc, ncut} vs oracle {ona, onc, oncut}"
+    act, oact = dm.downloadActiveNodes(), ox.active()
+    assert np.array_equal(act["nix"], oact["nix"]), f"{tag}: sorted order differs at {np.nonzero(act['nix'] != oact['nix'])[0][:5]}"
+    assert np.array_equal(act["clusterId"], oact["clusterId"]), f"{tag}: active labels differ"
+    nd, ond = dm.downloadPartition(), ox.ndata()
+    for f in ("nix", "prefixVolume", "prefixEdgeDiff", "offsetInCluster"):
+        assert np.array_equal(nd[f], ond[f]), f"{tag}: NodeData.{f} differs at {np.nonzero(nd[f] != ond[f])[0][:5]}"
+    ids, cuts = dm.readSweepCuts()
+    ocuts = ox.sweepcuts()
+    assert np.array_equal(ids, ocuts["clusterId"]), f"{tag}: sweep-cut labels differ"
+    assert np.array_equal(cuts["sparsity"].view(np.uint32), ocuts["sparsity"].view(np.uint32)), \
+        f"{tag}: sparsity differs at {np.nonzero(cuts['sparsity'] != ocuts['sparsity'])[0][:5]}"
+    assert np.array_equal(cuts["offset"], ocuts["offset"]), f"{tag}: cut offsets differ"
+
+
+def _compare_cut(dm, ox, tag):
+    assert np.array_equal(dm.downloadLabels(), ox.labels()), \
+        f"{tag}: labels differ at {np.nonzero(dm.downloadLabels() != ox.labels())[0][:8]}"
+    a, b = dm.readRandomWalkValuesF64(), ox.values()
+    assert np.array_equal(_bits(a), _bits(b)), f"{tag}: values differ at {np.nonzero(_bits(a) != _bits(b))[0][:5]}"
+    assert np.array_equal(dm.downloadDegrees(), ox.internal_degrees()), f"{tag}: internal degrees differ"
+    assert dm.counts()[1] == ox.counts()[1], f"{tag}: active clusters {dm.counts()[1]} vs {ox.counts()[1]}"
+    oact = ox.active()
+    oact = oact[oact["clusterId"] >= 0]
+    act = dm.downloadActiveNodes()
+    assert np.array_equal(act["nix"], oact["nix"]) and np.array_equal(act["clusterId"], oact["clusterId"]), \
+        f"{tag}: active list after cut differs"
+
+
+@pytest.mark.parametrize("name", list(graph_cases()))
+def test_iteration_lockstep(name):
+    g = graph_cases()[name]()
+    dm = _engine(g)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    for it in range(1, 61):
+        if ox.counts()[1] == 0:
+            break
+        dm.iterateRandomWalk(); ox.walk()
+        dm.computeSweepCuts(); ox.sweep()
+        _compare_sweep(dm, ox, f"{name} iter {it} sweep")
+        dm.cutClusters(); ox.cut()
+        _compare_cut(dm, ox, f"{name} iter {it} cut")
+    assert ox.errors() == 0
+    assert dm.getNumCutEdges() == ox.num_cut_edges()
+    ag = dm.downloadActiveEdgeMap()
+    assert np.array_equal(ag, (ox.neighbors() != px.INVALID_EDGE).astype(np.int32))
+    assert np.array_equal(dm.downloadGraph().edges, ox.neighbors())
+
+
+@pytest.mark.parametrize("name", list(graph_cases()))
+def test_end_to_end(name):
+    g = graph_cases()[name]()
+    dm = _engine(g)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    it = dm.expanderDecomposition()
+    oit = ox.run()
+    assert it == oit
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids)
+    cut, vol, ivol, cond = px.verify_partition(g, ids, k)
+    assert vol.sum() == 2 * g.m and ivol.sum() + 2 * cut == 2 * g.m
+
+
+def test_batch_equals_solo():
+    graphs = [px.sbm(300 + 37 * i, 3 + i % 4, 0.06, 0.003, 100 + i) for i in range(7)]
+    graphs.insert(3, px.from_edges(50, np.arange(49), np.arange(1, 50)))
+    dm = px.DeviceManager()
+    dm.initialize_batch(graphs)
+    dm.expanderDecomposition()
+    ids, ks = dm.getFinalPartition()
+    its = dm.graphIterations()
+    off = 0
+    for i, g in enumerate(graphs):
+        solo = _engine(g)
+        sit = solo.expanderDecomposition()
+        sids, sk = solo.getFinalPartition()
+        assert its[i] == sit, f"graph {i}: iterations {its[i]} vs solo {sit}"
+        assert ks[i] == sk and np.array_equal(ids[off:off + g.n], sids), f"graph {i}: partition differs"
+        off += g.n

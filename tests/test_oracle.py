@@ -1,0 +1,350 @@
+"""Pins the oracle (CPU restatement of the reference's CUDA decomposition) to
+things it does not itself define: golden vectors of the reference's own CPU
+code, libstdc++'s minstd_rand, scipy's erfcinv, exact rational arithmetic,
+closed forms of the lazy walk and brute-force sweep cuts."""
+import json
+import os
+import random
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy.special import erfcinv
+
+import parex_amd as px
+from oracle_ctypes import Oracle, cluster_sum, fixsum, init_vector, lib, pack_key
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _graphs():
+    return {
+        "sbm_c1": px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582),
+        "torus16": px.torus(16, 5, 1),
+        "path50": px.from_edges(50, np.arange(49), np.arange(1, 50)),
+        "sbm500": px.sbm(500, 4, 0.05, 0.003, 3),
+    }
+
+
+# ------------------------------------------------------------------ init vector
+
+def test_minstd_matches_libstdcxx():
+    d = json.load(open(os.path.join(GOLD, "minstd.json")))
+    got = [lib().ox_minstd_first(s & 0xFFFFFFFF) for s in d["seeds"]]
+    assert got == d["first"]
+
+
+def test_hash_known_answers():
+    # cuda/devRandomWalk.h:29-34 evaluated with Python integers
+    def h(x):
+        x = (((x >> 16) ^ x) * 0x45D9F3B) & 0xFFFFFFFF
+        x = (((x >> 16) ^ x) * 0x45D9F3B) & 0xFFFFFFFF
+        return ((x >> 16) ^ x) & 0xFFFFFFFF
+
+    rnd = random.Random(5)
+    for x in [0, 1, 6582, 0xFFFFFFFF] + [rnd.getrandbits(32) for _ in range(200)]:
+        assert lib().ox_hash(x) == h(x)
+
+
+def test_erfcinv_matches_scipy():
+    ys = np.concatenate([np.linspace(1e-9, 1.999999, 2001), [1e-300, 4.66e-10, 1.0, 0.5, 1.5]])
+    for y in ys:
+        a, b = lib().ox_erfcinv(float(y)), float(erfcinv(y))
+        assert abs(a - b) <= 1e-13 * max(1.0, abs(b)), (y, a, b)
+
+
+def test_init_vector_moments_and_symmetry():
+    deg = np.full(200000, 9, np.uint32)
+    x = init_vector(deg, 6582)
+    z = x / 3.0
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+    assert np.array_equal(x.astype(np.float32).astype(np.float64), x)  # float values stored as double
+    assert np.all(init_vector(np.zeros(10, np.uint32), 1) == 0.0)     # sigma = sqrt(0)
+
+
+def test_init_value_formula_against_scipy():
+    # normal_distribution_nvcc<float>::sample with u from minstd, erfcinv from scipy
+    S1 = np.float32(1.0 / 2147483645.0)
+    for idx in range(64):
+        for deg in (1, 4, 13, 1000):
+            s = lib().ox_hash(6582 ^ idx)
+            u = lib().ox_minstd_first(s) - 1
+            S3 = np.float32(-1.4142135623730951)
+            if u > 2147483645 // 2:
+                u = 2147483645 - u
+                S3 = -S3
+            p = np.float32(Fraction(float(np.float32(u))) * Fraction(float(S1)) + Fraction(float(S1 / 2)))
+            e = np.float32(erfcinv(float(np.float32(2.0) * p)))
+            want = np.float32(np.float32(np.sqrt(float(deg))) * S3) * e
+            got = np.float32(lib().ox_init_value(idx, deg, 6582))
+            assert abs(int(got.view(np.int32)) - int(want.view(np.int32))) <= 1, (idx, deg, got, want)
+
+
+# ------------------------------------------------------------------ keys and exact sums
+
+def test_pack_key_orders_like_label_then_float():
+    rnd = np.random.default_rng(3)
+    vals = np.concatenate([rnd.normal(size=300).astype(np.float32), np.float32([0.0, -0.0, 1e-38, -1e-38, 3e38])])
+    labs = rnd.integers(-5, 5, size=len(vals))
+    keys = [pack_key(int(l), float(v)) for l, v in zip(labs, vals)]
+    order = sorted(range(len(vals)), key=lambda i: keys[i])
+    for a, b in zip(order, order[1:]):
+        la, lb = labs[a], labs[b]
+        assert la < lb or (la == lb and (vals[a] < vals[b] or (vals[a] == vals[b] and
+                                                               (np.signbit(vals[a]) >= np.signbit(vals[b])))))
+
+
+def _exact_fixsum(terms):
+    S = 0
+    for t in terms:
+        q = Fraction(float(t)) * (1 << 72)
+        S += int(q)                      # truncation toward zero
+    return float(Fraction(S, 1 << 72))   # one correct rounding
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fixsum_is_exact_and_order_free(seed):
+    rnd = np.random.default_rng(seed)
+    t = np.concatenate([rnd.normal(scale=10.0 ** rnd.integers(-12, 6), size=200),
+                        [1e-25, -3e-23, 2.0 ** 30, -(2.0 ** 30), 0.0]])
+    v, ovf = fixsum(t)
+    assert ovf == 0
+    assert v == _exact_fixsum(t)
+    assert fixsum(t[::-1])[0] == v and fixsum(rnd.permutation(t))[0] == v
+    assert fixsum(np.array([2.0 ** 31]))[1] == 1          # range guard
+
+
+def _exact_cluster_sum(vals):
+    vals = np.asarray(vals, np.float32)
+    amax = float(np.max(np.abs(vals)))
+    if amax == 0.0:
+        return np.float32(0.0)
+    E = int(np.frexp(amax)[1])
+    L = (len(vals) - 1).bit_length()
+    K = 62 - E - L
+    S = sum(int(Fraction(float(v)) * Fraction(2) ** K) for v in vals)
+    return np.float32(float(S) * 2.0 ** (-K))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_cluster_sum_definition(seed):
+    rnd = np.random.default_rng(seed)
+    v = rnd.normal(scale=10.0 ** rnd.integers(-8, 4), size=int(rnd.integers(1, 3000))).astype(np.float32)
+    got = cluster_sum(v)
+    assert np.float32(got) == _exact_cluster_sum(v)
+    assert cluster_sum(rnd.permutation(v)) == got
+    # close to the exactly rounded float sum
+    exact = float(sum(Fraction(float(a)) for a in v))
+    assert abs(got - exact) <= 1e-6 * float(np.abs(v).sum()) + 1e-30
+
+
+# ------------------------------------------------------------------ walk closed forms
+
+def test_walk_cycle_eigenvector():
+    n, k = 64, 3
+    g = px.from_edges(n, np.arange(n), (np.arange(n) + 1) % n)
+    x = np.cos(2 * np.pi * k * np.arange(n) / n)
+    ox = Oracle(g, init_values=x)
+    lam = 0.1 + 0.9 * np.cos(2 * np.pi * k / n)
+    for step in range(1, 6):
+        ox.walk()
+        np.testing.assert_allclose(ox.values(), lam ** step * x, atol=1e-13)
+
+
+def test_walk_fixed_point_on_regular_graph_and_mass():
+    g = px.torus(8, 0, 1)                 # 4-regular
+    ox = Oracle(g, init_values=np.full(64, 2.5))
+    ox.walk()
+    assert np.all(ox.values() == 2.5)
+    h = px.sbm(300, 3, 0.1, 0.01, 4)
+    x0 = np.random.default_rng(0).normal(size=300)
+    ox = Oracle(h, init_values=x0)
+    deg = h.degree()
+    mass0 = x0[deg > 0].sum()
+    for _ in range(5):
+        ox.walk()
+    v = ox.values()
+    assert abs(v[deg > 0].sum() - mass0) < 1e-9 * np.abs(x0).sum()
+
+
+@pytest.mark.parametrize("name", ["sbm_c1", "torus16", "path50", "sbm500"])
+def test_walk_matches_reference_cpu(name):
+    gold = np.load(os.path.join(GOLD, "ref_cpu.npz"))
+    g = _graphs()[name]
+    ox = Oracle(g, init_values=init_vector(g.degree(), 6582))
+    done = 0
+    for k in (1, 4, 16):
+        while done < k:
+            ox.walk()
+            done += 1
+        # tolerance of tests/cudaTest.cpp:67 (CPU float push walk vs device walk)
+        np.testing.assert_allclose(ox.values(), gold[f"{name}_walk{k}"], atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("name", ["sbm_c1", "torus16", "path50", "sbm500"])
+def test_sweep_matches_reference_cpu(name):
+    gold = np.load(os.path.join(GOLD, "ref_cpu.npz"))
+    g = _graphs()[name]
+    for k in (1, 3, 10):
+        ox = Oracle(g, init_values=init_vector(g.degree(), 6582))
+        for _ in range(k):
+            ox.walk()
+        ox.sweep()
+        sc = ox.sweepcuts()
+        assert len(sc) == 1 and sc[0]["clusterId"] == 0
+        ref_sp = gold[f"{name}_sweep{k}"][0]
+        ref_off = int(gold[f"{name}_sweep{k}_offset"][0])
+        assert sc[0]["sparsity"] == ref_sp
+        # all prefix conductances from the oracle's NodeData: the reference keeps
+        # the leftmost minimum (0-based), the device path the rightmost (1-based)
+        nd = ox.ndata()
+        tot = int(nd["prefixVolume"][-1])
+        sp = []
+        for e, v in zip(nd["prefixEdgeDiff"].astype(np.int64), nd["prefixVolume"].astype(np.int64)):
+            den = 0 if v >= tot else min(v, tot - v)
+            sp.append(np.float32(np.float32(np.uint32(e & 0xFFFFFFFF)) / np.float32(den)) if den else np.float32(2.0))
+        sp = np.array(sp, np.float32)
+        mins = np.nonzero(sp == sp.min())[0]
+        assert sp.min() == ref_sp
+        assert mins[0] == ref_off and mins[-1] + 1 == sc[0]["offset"]
+
+
+def test_verify_matches_reference_cpu():
+    gold = np.load(os.path.join(GOLD, "ref_cpu.npz"))
+    for name, g in _graphs().items():
+        rng = np.random.default_rng(11)
+        for parts, ref_k, ref_cut in gold[f"{name}_verify"]:
+            ids = rng.integers(0, parts, g.n).astype(np.int32)
+            cut, vol, ivol, _ = px.verify_partition(g, ids, int(parts))
+            assert cut == ref_cut and int(parts) == ref_k
+            assert lib().ox_verify_cut_edges(g.n, g.ranges, g.edges, ids) == ref_cut
+            assert vol.sum() == 2 * g.m and ivol.sum() + 2 * cut == 2 * g.m
+
+
+# ------------------------------------------------------------------ sweep brute force
+
+def _brute_sweep(g, x, nix_order):
+    """Best (sparsity, rightmost 1-based offset) of one cluster by enumeration."""
+    keys = [(pack_key(0, float(np.float32(x[u]))), i, u) for i, u in enumerate(nix_order)]
+    order = [u for _, _, u in sorted(keys)]
+    deg = g.degree()
+    tot = int(deg.sum())
+    best = (np.float32(np.inf), 0)
+    for p in range(1, len(order) + 1):
+        S = set(order[:p])
+        cut = sum(1 for u in S for v in g.edges[g.ranges[u]:g.ranges[u + 1]] if v not in S)
+        vol = int(sum(deg[u] for u in S))
+        den = 0 if vol >= tot else min(vol, tot - vol)
+        sp = np.float32(np.float32(cut) / np.float32(den)) if den else np.float32(2.0)
+        if sp <= best[0]:
+            best = (sp, p)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_sweep_brute_force_small(seed):
+    rnd = np.random.default_rng(seed)
+    n = int(rnd.integers(4, 11))
+    pairs = [(a, b) for a in range(n) for b in range(a + 1, n) if rnd.random() < 0.45]
+    pairs += [(i, i + 1) for i in range(n - 1)]
+    pairs = sorted(set(pairs))
+    g = px.from_edges(n, [a for a, _ in pairs], [b for _, b in pairs])
+    x0 = rnd.normal(size=n)
+    x0[rnd.integers(0, n)] = x0[0]          # force a tie
+    ox = Oracle(g, init_values=x0)
+    ox.walk()
+    ox.sweep()
+    sc = ox.sweepcuts()[0]
+    sp, off = _brute_sweep(g, ox.values(), list(range(n)))
+    assert sc["sparsity"] == sp and sc["offset"] == off
+
+
+def test_barbell_cut():
+    # two 6-cliques joined by one edge: the sweep must find the bridge,
+    # conductance 1 / (6*5 + 1)
+    e = [(a, b) for a in range(6) for b in range(a + 1, 6)] + [(a + 6, b + 6) for a in range(6) for b in range(a + 1, 6)]
+    e.append((5, 6))
+    g = px.from_edges(12, [a for a, _ in e], [b for _, b in e])
+    x0 = np.r_[np.full(6, -1.0), np.full(6, 1.0)] + np.linspace(0, 1e-3, 12)
+    ox = Oracle(g, init_values=x0)
+    ox.walk()
+    ox.sweep()
+    sc = ox.sweepcuts()[0]
+    assert sc["offset"] == 6 and sc["sparsity"] == np.float32(1.0) / np.float32(31.0)
+    ox.cut()
+    lab = ox.labels()
+    assert len(set(lab[:6])) == 1 and len(set(lab[6:])) == 1 and lab[0] != lab[6]
+
+
+# ------------------------------------------------------------------ full runs and invariants
+
+@pytest.mark.parametrize("name", ["torus16", "path50", "sbm500"])
+def test_oracle_run_invariants(name):
+    g = _graphs()[name]
+    ox = Oracle(g)
+    it = ox.run()
+    assert ox.errors() == 0 and it >= 2
+    ids, k = ox.final_partition()
+    assert ids.min() == 0 and ids.max() == k - 1 and len(np.unique(ids)) == k
+    cut, vol, ivol, _ = px.verify_partition(g, ids, k)
+    assert ivol.sum() + 2 * cut == 2 * g.m
+    # every still-valid arc joins two nodes of one cluster (Q12: rows of
+    # deactivated nodes keep their arcs, so check from surviving rows only)
+    nb = ox.neighbors()
+    lab = ox.labels()
+    valid_rows = 0
+    for u in range(g.n):
+        if lab[u] < 0:
+            continue
+        for j in range(g.ranges[u], g.ranges[u + 1]):
+            if nb[j] != px.INVALID_EDGE:
+                assert lab[nb[j]] == lab[u]
+                valid_rows += 1
+    # internal degrees equal the surviving arcs
+    dint = ox.internal_degrees()
+    for u in range(g.n):
+        assert dint[u] == sum(1 for j in range(g.ranges[u], g.ranges[u + 1]) if nb[j] != px.INVALID_EDGE)
+
+
+def test_thresholds_extremes():
+    g = px.sbm(300, 3, 0.1, 0.01, 4)
+    # sc_threshold 0: never cut; every cluster ends when its walk converges
+    ox = Oracle(g, sc=0.0)
+    ox.run()
+    ids, k = ox.final_partition()
+    cc = _components(g)
+    assert k <= cc + 1
+    # huge rw_threshold: everything is certified in iteration 1
+    ox = Oracle(g, rw=1e30)
+    assert ox.run() == 2
+
+
+def _components(g):
+    seen = np.zeros(g.n, bool)
+    c = 0
+    for s in range(g.n):
+        if seen[s]:
+            continue
+        c += 1
+        st = [s]
+        seen[s] = True
+        while st:
+            u = st.pop()
+            for v in g.edges[g.ranges[u]:g.ranges[u + 1]]:
+                if not seen[v]:
+                    seen[v] = True
+                    st.append(v)
+    return c
+
+
+def test_connected_components_min_index_labels():
+    # two paths and isolated nodes: after iteration 1 every component carries
+    # its minimum node index before recentring relabels it
+    g = px.from_edges(12, [0, 1, 2, 5, 6, 9], [1, 2, 3, 6, 7, 10])
+    ox = Oracle(g, rw=1e30, sc=0.0)
+    ox.walk(); ox.sweep(); ox.cut()
+    lab = ox.labels()
+    # all deactivated (rw huge): labels are smallest(0) - index - 1 in min-index order
+    comps = [[0, 1, 2, 3], [4], [5, 6, 7], [8], [9, 10], [11]]
+    for i, comp in enumerate(comps):
+        assert set(lab[comp]) == {-i - 1}, (comp, lab[comp])
