@@ -1,0 +1,198 @@
+"""Benchmark: expander decomposition on one MI355X, batched vs plain.
+
+A step is one complete decomposition of the workload's graph set (every graph
+run to its final partition).  Inputs are resident in HBM when a step starts:
+`reset` (untimed) restores the engine from the device copy of the input.
+
+Default workload (config 4 of the benchmark plan): 512 independent planted
+partition graphs, n log-uniform in [1K, 256K], k ~ U{4..64}, intra degree
+~ U(8, 32), inter degree ~ U(0.5, 3), seed 11; parameters seed 6582,
+sweep threshold 0.3, walk threshold 1e-4.
+
+Printed (one JSON line on stdout): batched edges/s as `value`, the plain
+one-graph-per-engine path, their ratio, the walk kernel's HBM roofline and a
+CPU baseline (the reference's single-threaded CPU decomposition, oracle/_ref,
+on a bounded sample of the same graphs).
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import parex_amd as px  # noqa: E402
+
+METRIC = "edges decomposed/sec, batched vs plain (1 GPU) + p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name, graphs_limit=None):
+    if name == "c4":
+        specs = px.sbm_specs(512, 1024, 262144, 4, 64, (8.0, 32.0), (0.5, 3.0), 11)
+        if graphs_limit:
+            specs = specs[:graphs_limit]
+        gs = px.sbm_batch(specs)
+        desc = {"workload": "512 planted-partition SBMs, n log-uniform [1K,256K], k U{4..64}, "
+                            "d_in U(8,32), d_out U(0.5,3), block-diagonal batch",
+                "graphs": len(gs), "seed": 11}
+        return gs, desc
+    if name == "c2":
+        g = px.torus(1024, 10486, 1)
+        return [g], {"workload": "torus 1024x1024 + 0.5% random long-range edges", "graphs": 1, "seed": 1}
+    if name == "c1":
+        g = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
+        return [g], {"workload": "SBM n=4096, 16 blocks, p_in=12/255, p_out=1/3840", "graphs": 1, "seed": 6582}
+    if name == "c3":
+        g = px.rmat(22, 16, 0.57, 0.19, 0.19, 7)
+        return [g], {"workload": "RMAT scale 22, edge factor 16 (.57,.19,.19,.05), symmetrised", "graphs": 1,
+                     "seed": 7}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(graphs, budget_s):
+    """Reference single-threaded CPU decomposition (oracle/_ref) on a bounded
+    prefix of the workload; falls back to the oracle's restatement."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "parex_cpu_ref")
+    with tempfile.TemporaryDirectory() as td:
+        paths, edges = [], 0
+        for i, g in enumerate(graphs):
+            if edges > 40e6:
+                break
+            p = os.path.join(td, f"g{i}.pxg")
+            px.write_pxg(p, g)
+            paths.append(p)
+            edges += g.m
+        if os.path.exists(ref):
+            cmd = [ref, "decompose", str(budget_s), "6582", "0.3", "1e-4", *paths]
+            if shutil.which("taskset"):
+                cmd = ["taskset", "-c", "0", *cmd]
+            out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=budget_s * 4 + 60).stdout
+            r = json.loads(out)
+            return {"value": r["edges"] / r["seconds"], "unit": "edges/s", "cores": 1, "kind": "reference",
+                    "sample": f"first {r['graphs']} graphs of the workload ({r['edges']} edges), "
+                              f"lib/algorithms/expander_decomposition.h with Config{{0.3,1e-4,1,1}}, "
+                              f"{r['seconds']:.2f} s on 1 core"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ctypes import Oracle
+    t0, e, k = time.perf_counter(), 0, 0
+    for g in graphs:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        Oracle(g).run()
+        e += g.m
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": e / dt, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"first {k} graphs ({e} edges) through the oracle, {dt:.2f} s on 1 core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c4")
+    ap.add_argument("--graphs", type=int, default=None, help="use only the first G graphs of the set")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+              file=sys.stderr)
+        return 2
+
+    t = time.perf_counter()
+    graphs, cfg = workload(a.workload, a.graphs)
+    n_tot = sum(g.n for g in graphs)
+    m_tot = sum(g.m for g in graphs)
+    cfg.update({"nodes": n_tot, "edges": m_tot, "params": {"seed": 6582, "sc_threshold": 0.3, "rw_threshold": 1e-4}})
+    log(f"workload {a.workload}: {len(graphs)} graphs, {n_tot} nodes, {m_tot} edges ({time.perf_counter() - t:.1f} s)")
+
+    # ---------------------------------------------------------------- batched
+    dm = px.DeviceManager(px.Params(keep_original=True, record_node_data=False))
+    t = time.perf_counter()
+    dm.initialize_batch(graphs)
+    log(f"batched initialize {time.perf_counter() - t:.2f} s")
+    for w in range(a.warmup):
+        dm.reset()
+        it = dm.expanderDecomposition()
+        log(f"warmup {w}: {dm.lastRunMs():.1f} ms, iterations {it}")
+    step_ms = []
+    for k in range(a.steps):
+        dm.reset()
+        t = time.perf_counter()
+        it = dm.expanderDecomposition()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        log(f"step {k}: {step_ms[-1]:.1f} ms wall, {dm.lastRunMs():.1f} ms device, iterations {it}")
+    ms = float(np.mean(step_ms))
+    value = m_tot / (ms / 1e3)
+    batch_ids, batch_k = dm.getFinalPartition()
+    batch_it = dm.graphIterations()
+
+    # ---------------------------------------------------------------- walk roofline
+    dm.reset()
+    dm.enableWalkTiming(True)
+    dm.expanderDecomposition()
+    launches, walk_ms, walk_bytes = dm.walkStats()
+    dm.enableWalkTiming(False)
+    achieved = walk_bytes / (walk_ms / 1e3) / 1e9 if walk_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_walk", "launches": launches,
+                "avg_launch_us": round(walk_ms * 1e3 / max(launches, 1), 2),
+                "alg_bytes_per_launch": round(walk_bytes / max(launches, 1))}
+    dm.close()
+
+    # ---------------------------------------------------------------- plain
+    plain = None
+    same = None
+    if not a.no_plain:
+        tot = 0.0
+        same = True
+        off = 0
+        for i, g in enumerate(graphs):
+            d1 = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
+            d1.initialize(g)
+            t = time.perf_counter()
+            it1 = d1.expanderDecomposition()
+            tot += time.perf_counter() - t
+            ids1, k1 = d1.getFinalPartition()
+            if a.workload == "c4" or len(graphs) > 1:
+                if not (np.array_equal(ids1, batch_ids[off:off + g.n]) and k1 == batch_k[i] and it1 == batch_it[i]):
+                    same = False
+            d1.close()
+            off += g.n
+        plain = m_tot / tot
+        log(f"plain: {tot * 1e3:.1f} ms total, {plain:.3e} edges/s; batched == plain: {same}")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None if a.no_cpu else cpu_baseline(graphs, a.cpu_budget)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": cfg,
+        "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
+        "batched_equals_plain": same,
+        "iterations_max": int(batch_it.max()) if len(batch_it) else 0,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -74,6 +74,16 @@ int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* nod
                          const uint32_t* edge_counts, const uint32_t* const* ranges_ptrs,
                          const uint32_t* const* edges_ptrs);
 
+/* The reference reads sc_threshold / rw_threshold from globals on every step
+ * call (cuda/devSweepCut.h:701, cuda/devPartition.h:555, 632); callers that
+ * change them between steps update the engine with this call. */
+int pxd_set_thresholds(pxd_engine* e, float sc_threshold, float rw_threshold);
+
+/* Return to the state pxd_initialize / pxd_initialize_batch left, from the
+ * device-resident copy of the input (needs keep_original = 1).  Equivalent to
+ * calling initialize again with the same graphs, without the host transfer. */
+int pxd_reset(pxd_engine* e);
+
 /* iterateRandomWalk()   (cuda/interface.h:29, cuda/device.h:40-48) */
 int pxd_iterate_random_walk(pxd_engine* e);
 /* computeSweepCuts()    (cuda/interface.h:32, cuda/device.h:77-81) */

@@ -109,8 +109,9 @@ def pxd() -> C.CDLL:
         lib.pxd_initialize.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p]
         lib.pxd_initialize_batch.argtypes = [P, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_void_p),
                                              C.POINTER(C.c_void_p)]
-        for f in ("pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters"):
+        for f in ("pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters", "pxd_reset"):
             getattr(lib, f).argtypes = [P]
+        lib.pxd_set_thresholds.argtypes = [P, C.c_float, C.c_float]
         lib.pxd_expander_decomposition.argtypes = [P, C.POINTER(C.c_int32)]
         lib.pxd_graph_iterations.argtypes = [P, _i32p, C.c_uint32]
         lib.pxd_get_final_partition.argtypes = [P, _i32p, _i32p]
@@ -136,7 +137,7 @@ def pxd() -> C.CDLL:
 # C-ABI entry points declared in include/parex_amd/pxd.h (checked by the tests)
 PXD_SYMBOLS = [
     "pxd_default_params", "pxd_last_error", "pxd_create", "pxd_destroy", "pxd_initialize",
-    "pxd_initialize_batch", "pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters",
+    "pxd_initialize_batch", "pxd_set_thresholds", "pxd_reset", "pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters",
     "pxd_expander_decomposition", "pxd_graph_iterations", "pxd_get_final_partition",
     "pxd_read_random_walk_values", "pxd_read_random_walk_values_f64", "pxd_download_degrees",
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
@@ -313,6 +314,13 @@ class DeviceManager:
         self._n, self._m, self._graphs = int(nc.sum()), int(ec.sum()), G
         self._node_counts = nc
         self._check(self._lib.pxd_initialize_batch(self._h, G, nc, ec, rp, ep))
+
+    def setThresholds(self, sc_threshold: float, rw_threshold: float):
+        self.params.sc_threshold, self.params.rw_threshold = sc_threshold, rw_threshold
+        self._check(self._lib.pxd_set_thresholds(self._h, sc_threshold, rw_threshold))
+
+    def reset(self):
+        self._check(self._lib.pxd_reset(self._h))
 
     def iterateRandomWalk(self):
         self._check(self._lib.pxd_iterate_random_walk(self._h))

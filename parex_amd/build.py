@@ -51,12 +51,13 @@ def build_engine():
 
 
 def build_cli():
-    srcs = [os.path.join(CSRC, "facade.cpp"), os.path.join(CSRC, "cli.cpp")]
-    if not all(os.path.exists(s) for s in srcs):
-        return
+    """libparex_cuda.so: the CudaDeviceManager facade; bin/parex: the CLI."""
+    _run(["g++", "-O2", "-std=c++20", "-Wall", "-fPIC", "-shared", f"-I{INC}", os.path.join(CSRC, "facade.cpp"),
+          "-o", os.path.join(PKG, "libparex_cuda.so"), f"-L{PKG}", "-lpxd", "-Wl,-rpath,$ORIGIN"])
     os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
-    _run(["g++", "-O2", "-std=c++20", "-Wall", f"-I{INC}", *srcs, "-o", os.path.join(PKG, "bin", "parex"),
-          f"-L{PKG}", "-lpxd", "-lpxgen", "-Wl,-rpath,$ORIGIN/..", "-D__HIP_PLATFORM_AMD__"])
+    _run(["g++", "-O2", "-std=c++20", "-Wall", f"-I{INC}", os.path.join(CSRC, "cli.cpp"),
+          "-o", os.path.join(PKG, "bin", "parex"), f"-L{PKG}", "-lparex_cuda", "-lpxd", "-lpxgen",
+          "-Wl,-rpath,$ORIGIN/.."])
 
 
 def build_oracle():

@@ -323,6 +323,16 @@ struct pxd_engine {
         return init_state();
     }
 
+    // back to the state right after initialize, from the device-resident input CSR
+    int reset() {
+        if (!inited && h_gnode.empty()) return set_err(PXD_ERR_ARG, "engine not initialized");
+        if (!P.keep_original) return set_err(PXD_ERR_ARG, "reset needs keep_original = 1");
+        HIPC(hipSetDevice(dev));
+        if (A) HIPC(hipMemcpyAsync(d_nbr, d_nbr0, (size_t)A * 4, hipMemcpyDeviceToDevice, st));
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
+        return init_state();
+    }
+
     // initial clusters: one per non-empty graph (label 0, connectivity unknown)
     int init_state() {
         std::vector<uint32_t> cs, cg, cmin, cvol, gclu(G, 0);
@@ -772,6 +782,15 @@ int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* nod
         return set_err(PXD_ERR_ARG, "null argument or empty batch");
     return e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
 }
+
+int pxd_set_thresholds(pxd_engine* e, float sc_threshold, float rw_threshold) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    e->P.sc_threshold = sc_threshold;
+    e->P.rw_threshold = rw_threshold;
+    return 0;
+}
+
+int pxd_reset(pxd_engine* e) { return e ? e->reset() : set_err(PXD_ERR_ARG, "null engine"); }
 
 int pxd_iterate_random_walk(pxd_engine* e) { return e ? e->walk() : set_err(PXD_ERR_ARG, "null engine"); }
 int pxd_compute_sweep_cuts(pxd_engine* e) { return e ? e->sweep() : set_err(PXD_ERR_ARG, "null engine"); }
