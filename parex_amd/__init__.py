@@ -202,8 +202,9 @@ def sbm_specs(count, nmin, nmax, kmin, kmax, din, dout, seed):
 
 def sbm_batch(specs) -> List[Graph]:
     count = len(specs)
+    arr = (_SbmSpec * count)(*specs)
     out = (_PxgGraph * count)()
-    _gen_check(pxgen().pxg_sbm_batch(count, specs, out))
+    _gen_check(pxgen().pxg_sbm_batch(count, arr, out))
     return [_take(out[i]) for i in range(count)]
 
 
