@@ -149,6 +149,23 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
 
+template <int G>
+__device__ __forceinline__ uint32_t group_min(uint32_t v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, G));
+    return v;
+}
+
+// Append slot for every lane with pred set: one atomic per wave, slots in lane
+// order.  All 64 lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t b = __ballot(pred);
+    uint32_t base = 0;
+    if (b != 0ull && lane_id() == 0) base = atomicAdd(counter, (uint32_t)__popcll(b));
+    base = __shfl(base, 0);
+    return base + (uint32_t)__popcll(b & ((1ull << lane_id()) - 1ull));
+}
+
 // Inclusive scan over the wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     uint32_t lane = lane_id();

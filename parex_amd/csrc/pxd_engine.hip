@@ -483,9 +483,9 @@ struct pxd_engine {
                                                                      d_xkey, d_ediff);
     }
     template <int GSZ>
-    void launch_hook(uint32_t nl) {
+    void launch_hook(uint32_t nl, uint32_t lim) {
         k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root,
-                                                                d_cnt);
+                                                                lim, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
@@ -653,20 +653,33 @@ struct pxd_engine {
 
     void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph);
 
+    // jump until every member points at its root
+    int cc_flatten(uint32_t nl) {
+        for (;;) {
+            HIPC(hipMemsetAsync(&d_cnt->jump_changed, 0, 4, st));
+            for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
+            HIPC(hipGetLastError());
+            CHK(read_counters());
+            if (!hc.jump_changed) return 0;
+        }
+    }
+
+    // Components of every CC part: one sampling round over the first two arcs
+    // of each node (clusters are expanders, so it links almost everything),
+    // then full rounds until a round on flat trees changes nothing.
     int connected_components(uint32_t nl) {
         if (nl == 0) return 0;
+        HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
+        PXD_DISPATCH_GS(launch_hook, nl, 2u);
+        HIPC(hipGetLastError());
+        CHK(cc_flatten(nl));
         for (int round = 0;; ++round) {
             HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
-            PXD_DISPATCH_GS(launch_hook, nl);
+            PXD_DISPATCH_GS(launch_hook, nl, 0xFFFFFFFFu);
             HIPC(hipGetLastError());
-            for (;;) {
-                HIPC(hipMemsetAsync(&d_cnt->jump_changed, 0, 4, st));
-                for (int j = 0; j < 4; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
-                HIPC(hipGetLastError());
-                CHK(read_counters());
-                if (!hc.jump_changed) break;
-            }
+            CHK(read_counters());
             if (!hc.hook_changed) break;
+            CHK(cc_flatten(nl));
             if (round > 100000) return set_err(PXD_ERR_STATE, "connected components did not converge");
         }
         return 0;

@@ -194,15 +194,21 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __
 
 // Sort classes: size <= 1 nothing, <= 64 one wave, <= TILE one block in LDS,
 // larger: LSD radix over 8-bit digits with per-tile histograms.
-__global__ void k_classify(uint32_t nseg, const uint32_t* __restrict__ segs,
-                           uint32_t* __restrict__ lwave, uint32_t* __restrict__ lblock,
-                           uint32_t* __restrict__ llarge, Counters* cnt) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nseg; c += gridDim.x * blockDim.x) {
-        const uint32_t s = segs[c + 1] - segs[c];
-        if (s <= 1) continue;
-        if (s <= WAVE) lwave[atomicAdd(&cnt->nwave, 1u)] = c;
-        else if (s <= (uint32_t)TILE) lblock[atomicAdd(&cnt->nblock, 1u)] = c;
-        else llarge[atomicAdd(&cnt->nlarge, 1u)] = c;
+__global__ void __launch_bounds__(BS) k_classify(uint32_t nseg, const uint32_t* __restrict__ segs,
+                                                 uint32_t* __restrict__ lwave, uint32_t* __restrict__ lblock,
+                                                 uint32_t* __restrict__ llarge, Counters* cnt) {
+    for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nseg; c0 += gridDim.x * blockDim.x) {
+        const uint32_t c = c0 + threadIdx.x;
+        const uint32_t s = c < nseg ? segs[c + 1] - segs[c] : 0u;
+        const bool w = s > 1u && s <= (uint32_t)WAVE;
+        const bool b = s > (uint32_t)WAVE && s <= (uint32_t)TILE;
+        const bool l = s > (uint32_t)TILE;
+        const uint32_t sw = wave_append(&cnt->nwave, w);
+        const uint32_t sb = wave_append(&cnt->nblock, b);
+        const uint32_t sl = wave_append(&cnt->nlarge, l);
+        if (w) lwave[sw] = c;
+        if (b) lblock[sb] = c;
+        if (l) llarge[sl] = c;
     }
 }
 
@@ -559,68 +565,89 @@ __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbe
 }
 
 // cuda/devPartition.h:545-622: positions beyond the cut offset form the second
-// part.  Members of CC clusters get a part id, a fresh root and join the CC list.
-__global__ void k_mark(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                       const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
-                       const uint32_t* __restrict__ coff, uint32_t iter, uint32_t* __restrict__ part,
-                       uint32_t* __restrict__ root, uint32_t* __restrict__ stamp,
-                       uint32_t* __restrict__ cclist, Counters* cnt) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
-        const uint32_t c = pclu[k];
-        const uint32_t f = cflag[c];
-        if (!(f & CF_CC)) continue;
-        const uint32_t u = pnix[k];
-        const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
-        part[u] = (c << 1) | side;
-        root[u] = u;
-        stamp[u] = iter;
-        cclist[atomicAdd(&cnt->cclist_n, 1u)] = u;
+// part.  Members of CC clusters get a part id, a fresh root and join the CC list
+// (one atomic per wave for the list append).
+__global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __restrict__ pnix,
+                                             const uint32_t* __restrict__ pclu,
+                                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
+                                             const uint32_t* __restrict__ coff, uint32_t iter,
+                                             uint32_t* __restrict__ part, uint32_t* __restrict__ root,
+                                             uint32_t* __restrict__ stamp, uint32_t* __restrict__ cclist,
+                                             Counters* cnt) {
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        bool member = false;
+        uint32_t u = 0;
+        if (k < na) {
+            const uint32_t c = pclu[k];
+            const uint32_t f = cflag[c];
+            if (f & CF_CC) {
+                member = true;
+                u = pnix[k];
+                const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
+                part[u] = (c << 1) | side;
+                root[u] = u;
+                stamp[u] = iter;
+            }
+        }
+        const uint32_t slot = wave_append(&cnt->cclist_n, member);
+        if (member) cclist[slot] = u;
     }
 }
 
-// Hooking round: for every same-part valid arc with different parent pointers,
-// atomicMin the larger pointer's slot to the smaller (parents only decrease,
-// so the root of a tree is its minimum node).  Any difference seen schedules
-// another round; the loop ends only when all arcs agree on flat trees.
+// Hooking round over the CC members, GS lanes per node: m = smallest parent
+// pointer among the node's same-part neighbours (only the first `lim` arcs in
+// the sampling round); if m is below the node's own pointer pu, the slot pu is
+// lowered to m.  Pointers only decrease, so a tree's root is its minimum node.
+// A round that finds no smaller neighbour pointer on flat trees proves every
+// part's components are single trees.
 template <int GS>
 __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ part,
-                                                uint32_t* __restrict__ root, Counters* cnt) {
+                                                uint32_t* __restrict__ root, uint32_t lim, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const uint32_t groups = gridDim.x * (BS / GS);
     uint32_t changed = 0;
     for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
         const uint32_t u = list[i];
-        const uint32_t start = row[u], d = dint[u];
+        const uint32_t start = row[u], d = min(dint[u], lim);
         const uint32_t pp = part[u];
         const uint32_t pu = root[u];
+        uint32_t m = pu;
         for (uint32_t j = lane; j < d; j += GS) {
             const uint32_t v = nbr[start + j];
-            if (part[v] != pp) continue;
-            const uint32_t pv = root[v];
-            if (pv != pu) {
-                atomicMin(&root[max(pu, pv)], min(pu, pv));
-                changed = 1;
-            }
+            if (part[v] == pp) m = min(m, root[v]);
+        }
+        m = group_min<GS>(m);
+        if (lane == 0 && m < pu) {
+            changed = 1;
+            if (root[pu] > m) atomicMin(&root[pu], m);
         }
     }
-    if (changed) atomicOr(&cnt->hook_changed, 1u);
+    if (__any(changed) && lane_id() == 0) atomicOr(&cnt->hook_changed, 1u);
 }
 
-// pointer jumping: root[u] <- root[root[u]]
-__global__ void k_cc_jump(uint32_t nl, const uint32_t* __restrict__ list, uint32_t* __restrict__ root,
-                          Counters* cnt) {
+// pointer jumping, up to 8 steps per node and launch
+__global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __restrict__ list,
+                                                uint32_t* __restrict__ root, Counters* cnt) {
     uint32_t changed = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < nl; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        if (i >= nl) continue;
         const uint32_t u = list[i];
-        const uint32_t r = root[u];
-        const uint32_t rr = root[r];
-        if (rr != r) { root[u] = rr; changed = 1; }
+        const uint32_t r0 = root[u];
+        uint32_t r = r0;
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t rr = root[r];
+            if (rr == r) break;
+            r = rr;
+        }
+        if (r != r0) { root[u] = r; changed = 1; }
     }
-    if (changed) atomicOr(&cnt->jump_changed, 1u);
+    if (__any(changed) && lane_id() == 0) atomicOr(&cnt->jump_changed, 1u);
 }
 
 // regroup key: component root for CC members, the cluster's minimum node otherwise
