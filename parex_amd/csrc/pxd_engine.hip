@@ -46,10 +46,14 @@ static int set_err(int code, const char* fmt, ...) {
     } while (0)
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+// Grid for grid-stride kernels: enough workgroups to fill 256 CUs many times
+// over, never more (each extra workgroup costs dispatch time and, for kernels
+// that end with a flag update, one more atomic on a shared word).
+constexpr uint64_t kMaxGrid = 8192;
 static inline uint32_t grid_for(uint64_t n, uint32_t per_block) {
     uint64_t g = (n + per_block - 1) / per_block;
     if (g < 1) g = 1;
-    if (g > (1u << 20)) g = 1u << 20;
+    if (g > kMaxGrid) g = kMaxGrid;
     return (uint32_t)g;
 }
 
@@ -573,12 +577,13 @@ struct pxd_engine {
         ClusterTable* T = &ct[ctc];
         uint32_t nl = 0;
         if (ncc > 0) {
-            CHK(zero_counter_fields());
             k_mark<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
-                                                    d_root, d_stamp, d_cclist, d_cnt);
+                                                    d_root, d_stamp, d_tA);
             HIPC(hipGetLastError());
-            CHK(read_counters());
-            nl = hc.cclist_n;
+            CHK(scan(d_tA, d_tB, na));
+            k_compact_u32<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_cclist);
+            HIPC(hipGetLastError());
+            CHK(read_u32(d_tB + na, &nl));
             CHK(connected_components(nl));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);

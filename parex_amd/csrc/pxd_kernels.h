@@ -565,33 +565,26 @@ __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbe
 }
 
 // cuda/devPartition.h:545-622: positions beyond the cut offset form the second
-// part.  Members of CC clusters get a part id, a fresh root and join the CC list
-// (one atomic per wave for the list append).
+// part.  Members of CC clusters get a part id and a fresh root; member[k]
+// flags their positions (the CC list is their scan-compacted node ids).
 __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __restrict__ pnix,
                                              const uint32_t* __restrict__ pclu,
                                              const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                                              const uint32_t* __restrict__ coff, uint32_t iter,
                                              uint32_t* __restrict__ part, uint32_t* __restrict__ root,
-                                             uint32_t* __restrict__ stamp, uint32_t* __restrict__ cclist,
-                                             Counters* cnt) {
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
-        const uint32_t k = k0 + threadIdx.x;
-        bool member = false;
-        uint32_t u = 0;
-        if (k < na) {
-            const uint32_t c = pclu[k];
-            const uint32_t f = cflag[c];
-            if (f & CF_CC) {
-                member = true;
-                u = pnix[k];
-                const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
-                part[u] = (c << 1) | side;
-                root[u] = u;
-                stamp[u] = iter;
-            }
+                                             uint32_t* __restrict__ stamp, uint32_t* __restrict__ member) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pclu[k];
+        const uint32_t f = cflag[c];
+        const bool m = (f & CF_CC) != 0u;
+        member[k] = m ? 1u : 0u;
+        if (m) {
+            const uint32_t u = pnix[k];
+            const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
+            part[u] = (c << 1) | side;
+            root[u] = u;
+            stamp[u] = iter;
         }
-        const uint32_t slot = wave_append(&cnt->cclist_n, member);
-        if (member) cclist[slot] = u;
     }
 }
 
@@ -627,7 +620,8 @@ __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __r
             if (root[pu] > m) atomicMin(&root[pu], m);
         }
     }
-    if (__any(changed) && lane_id() == 0) atomicOr(&cnt->hook_changed, 1u);
+    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->hook_changed, __ATOMIC_RELAXED) == 0u)
+        atomicOr(&cnt->hook_changed, 1u);
 }
 
 // pointer jumping, up to 8 steps per node and launch
@@ -647,7 +641,8 @@ __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __r
         }
         if (r != r0) { root[u] = r; changed = 1; }
     }
-    if (__any(changed) && lane_id() == 0) atomicOr(&cnt->jump_changed, 1u);
+    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->jump_changed, __ATOMIC_RELAXED) == 0u)
+        atomicOr(&cnt->jump_changed, 1u);
 }
 
 // regroup key: component root for CC members, the cluster's minimum node otherwise
