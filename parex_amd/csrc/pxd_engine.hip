@@ -118,6 +118,7 @@ struct pxd_engine {
     std::vector<uint32_t*> lvl_sum, lvl_scan;
     uint32_t* d_oneseg = nullptr;
     uint32_t* d_gclu = nullptr;
+    uint32_t* d_giant = nullptr;
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
     Counters hc{};
@@ -314,6 +315,7 @@ struct pxd_engine {
         }
         CHK(alloc(&d_oneseg, 2));
         CHK(alloc(&d_gclu, G));
+        CHK(alloc(&d_giant, 2 * n1 + 2));
         CHK(alloc(&d_u64, 1));
         CHK(alloc(&d_cnt, 1));
 
@@ -380,7 +382,8 @@ struct pxd_engine {
         iter1 = 0;
         sweep_done = false;
         const double avg = N ? (double)A / (double)N : 0.0;
-        gs = avg <= 6.0 ? 4 : avg <= 12.0 ? 8 : avg <= 24.0 ? 16 : avg <= 48.0 ? 32 : 64;
+        // lanes per node: about four arcs per lane (the arc loops keep four in flight)
+        gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
         inited = true;
         return 0;
     }
@@ -487,9 +490,9 @@ struct pxd_engine {
                                                                      d_xkey, d_ediff);
     }
     template <int GSZ>
-    void launch_hook(uint32_t nl, uint32_t lim) {
-        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root,
-                                                                lim, d_cnt);
+    void launch_hook(uint32_t nl) {
+        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_giant,
+                                                                d_root, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
@@ -584,7 +587,7 @@ struct pxd_engine {
             k_compact_u32<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_cclist);
             HIPC(hipGetLastError());
             CHK(read_u32(d_tB + na, &nl));
-            CHK(connected_components(nl));
+            CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
             HIPC(hipGetLastError());
@@ -669,18 +672,18 @@ struct pxd_engine {
         }
     }
 
-    // Components of every CC part: one sampling round over the first two arcs
-    // of each node (clusters are expanders, so it links almost everything),
-    // then full rounds until a round on flat trees changes nothing.
-    int connected_components(uint32_t nl) {
+    // Components of every CC part (clusters flagged CF_CC in table T): a
+    // sampling round over the first two arcs of each node, then full rounds
+    // that skip each part's giant tree, until a round changes nothing.
+    int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
-        HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
-        PXD_DISPATCH_GS(launch_hook, nl, 2u);
+        k_cc_sample<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root);
         HIPC(hipGetLastError());
         CHK(cc_flatten(nl));
         for (int round = 0;; ++round) {
+            k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
             HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
-            PXD_DISPATCH_GS(launch_hook, nl, 0xFFFFFFFFu);
+            PXD_DISPATCH_GS(launch_hook, nl);
             HIPC(hipGetLastError());
             CHK(read_counters());
             if (!hc.hook_changed) break;

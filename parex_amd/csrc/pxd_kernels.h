@@ -112,7 +112,20 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
         Fix128 acc = {0ull, 0ull};
-        for (uint32_t j = lane; j < d; j += GS) fix_add_term(acc, y[nbr[start + j]], ovf);
+        // four independent index loads, then four value gathers in flight per lane
+        for (uint32_t j = lane; j < d; j += 4 * GS) {
+            const uint32_t* p = nbr + start;
+            const bool b1 = j + GS < d, b2 = j + 2 * GS < d, b3 = j + 3 * GS < d;
+            const uint32_t v0 = p[j];
+            const uint32_t v1 = b1 ? p[j + GS] : v0;
+            const uint32_t v2 = b2 ? p[j + 2 * GS] : v0;
+            const uint32_t v3 = b3 ? p[j + 3 * GS] : v0;
+            const double y0 = y[v0], y1 = y[v1], y2 = y[v2], y3 = y[v3];
+            fix_add_term(acc, y0, ovf);
+            if (b1) fix_add_term(acc, y1, ovf);
+            if (b2) fix_add_term(acc, y2, ovf);
+            if (b3) fix_add_term(acc, y3, ovf);
+        }
         acc = group_sum_fix<GS>(acc);
         if (lane == 0) {
             if (d > (1u << 24)) ovf = 1;
@@ -145,11 +158,22 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
         const uint32_t start = row[u], d = dint[u];
         const uint32_t mk = xkey[u];
         int32_t c = 0;
-        for (uint32_t j = lane; j < d; j += GS) {
-            const uint32_t v = nbr[start + j];
-            const uint32_t vk = xkey[v];
-            const bool before = (vk != mk) ? (vk < mk) : (v < u);
-            c += before ? -1 : 1;
+        for (uint32_t j = lane; j < d; j += 4 * GS) {
+            const uint32_t* p = nbr + start;
+            uint32_t v[4], k[4];
+            bool b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                b[q] = j + q * GS < d;
+                v[q] = b[q] ? p[j + q * GS] : u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) k[q] = xkey[v[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool before = (k[q] != mk) ? (k[q] < mk) : (v[q] < u);
+                if (b[q]) c += before ? -1 : 1;
+            }
         }
         c = group_sum<GS>(c);
         if (lane == 0) ediff[u] = c;
@@ -588,40 +612,78 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
     }
 }
 
-// Hooking round over the CC members, GS lanes per node: m = smallest parent
-// pointer among the node's same-part neighbours (only the first `lim` arcs in
-// the sampling round); if m is below the node's own pointer pu, the slot pu is
-// lowered to m.  Pointers only decrease, so a tree's root is its minimum node.
-// A round that finds no smaller neighbour pointer on flat trees proves every
-// part's components are single trees.
+// Hooking rounds over the CC members.  For a same-part arc whose endpoints
+// carry different pointers a != b, slot max(a, b) is lowered to min(a, b) with
+// a plain store (racing stores keep every pointer below its node and inside
+// its component, so trees stay acyclic and a tree's root is its minimum node).
+// Sampling round: the first two arcs of every node.  Full rounds skip nodes
+// whose pointer equals their part's sampled giant root: every arc leaving the
+// giant tree has its other endpoint outside it and is hooked from that side.
+// A full round that finds no differing pointers on flat trees is final.
+__global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ row,
+                                                  const uint32_t* __restrict__ dint,
+                                                  const uint32_t* __restrict__ nbr,
+                                                  const uint32_t* __restrict__ part,
+                                                  uint32_t* __restrict__ root) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        const uint32_t start = row[u], d = min(dint[u], 2u);
+        const uint32_t pp = part[u];
+        for (uint32_t j = 0; j < d; ++j) {
+            const uint32_t v = nbr[start + j];
+            if (part[v] != pp) continue;
+            const uint32_t a = root[u], b = root[v];
+            if (a == b) continue;
+            const uint32_t hi = max(a, b), lo = min(a, b);
+            if (root[hi] > lo) root[hi] = lo;
+        }
+    }
+}
+
 template <int GS>
 __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ part,
-                                                uint32_t* __restrict__ root, uint32_t lim, Counters* cnt) {
+                                                const uint32_t* __restrict__ giant,
+                                                uint32_t* __restrict__ root, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const uint32_t groups = gridDim.x * (BS / GS);
     uint32_t changed = 0;
     for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
         const uint32_t u = list[i];
-        const uint32_t start = row[u], d = min(dint[u], lim);
         const uint32_t pp = part[u];
         const uint32_t pu = root[u];
-        uint32_t m = pu;
+        if (pu == giant[pp]) continue;                 // uniform within the group
+        const uint32_t start = row[u], d = dint[u];
         for (uint32_t j = lane; j < d; j += GS) {
             const uint32_t v = nbr[start + j];
-            if (part[v] == pp) m = min(m, root[v]);
-        }
-        m = group_min<GS>(m);
-        if (lane == 0 && m < pu) {
+            if (part[v] != pp) continue;
+            const uint32_t pv = root[v];
+            if (pv == pu) continue;
             changed = 1;
-            if (root[pu] > m) atomicMin(&root[pu], m);
+            const uint32_t hi = max(pu, pv), lo = min(pu, pv);
+            if (root[hi] > lo) root[hi] = lo;
         }
     }
     if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->hook_changed, __ATOMIC_RELAXED) == 0u)
         atomicOr(&cnt->hook_changed, 1u);
+}
+
+// giant-root candidate of every part: the pointer of the part's middle member
+__global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
+                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
+                           const uint32_t* __restrict__ root, uint32_t* __restrict__ giant) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t f = cflag[c];
+        if (!(f & CF_CC)) continue;
+        const uint32_t b = cs[c], len = cs[c + 1] - b;
+        const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
+        giant[c << 1] = o ? root[pnix[b + o / 2]] : INVALID;
+        giant[(c << 1) | 1] = (len > o) ? root[pnix[b + o + (len - o) / 2]] : INVALID;
+    }
 }
 
 // pointer jumping, up to 8 steps per node and launch
