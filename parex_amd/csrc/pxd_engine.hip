@@ -119,6 +119,8 @@ struct pxd_engine {
     uint32_t* d_oneseg = nullptr;
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
+    uint32_t* d_unres = nullptr;
+    uint32_t* d_cclist2 = nullptr;
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
     Counters hc{};
@@ -316,6 +318,8 @@ struct pxd_engine {
         CHK(alloc(&d_oneseg, 2));
         CHK(alloc(&d_gclu, G));
         CHK(alloc(&d_giant, 2 * n1 + 2));
+        CHK(alloc(&d_unres, 2 * n1 + 2));
+        CHK(alloc(&d_cclist2, N));
         CHK(alloc(&d_u64, 1));
         CHK(alloc(&d_cnt, 1));
 
@@ -490,9 +494,8 @@ struct pxd_engine {
                                                                      d_xkey, d_ediff);
     }
     template <int GSZ>
-    void launch_hook(uint32_t nl) {
-        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_giant,
-                                                                d_root, d_cnt);
+    void launch_hook(const uint32_t* list, uint32_t nl) {
+        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
@@ -661,33 +664,44 @@ struct pxd_engine {
 
     void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph);
 
-    // jump until every member points at its root
-    int cc_flatten(uint32_t nl) {
+    // jump until every listed member points at its root
+    int cc_flatten(const uint32_t* list, uint32_t nl) {
         for (;;) {
             HIPC(hipMemsetAsync(&d_cnt->jump_changed, 0, 4, st));
-            for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
+            for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, list, d_root, d_cnt);
             HIPC(hipGetLastError());
             CHK(read_counters());
             if (!hc.jump_changed) return 0;
         }
     }
 
-    // Components of every CC part (clusters flagged CF_CC in table T): a
-    // sampling round over the first two arcs of each node, then full rounds
-    // that skip each part's giant tree, until a round changes nothing.
+    // Components of every CC part (clusters flagged CF_CC in table T):
+    // sampling over the first four arcs of each node, flattening, then the
+    // one-root certificate; only members of parts that fail it go through the
+    // exact hooking rounds.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
-        k_cc_sample<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root);
+        k_cc_sample<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
         HIPC(hipGetLastError());
-        CHK(cc_flatten(nl));
+        CHK(cc_flatten(d_cclist, nl));
+        k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant,
+                                                    d_unres);
+        k_cc_check<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_unres);
+        k_cc_flags<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_unres, d_tC);
+        HIPC(hipGetLastError());
+        CHK(scan(d_tC, d_tA, nl));
+        uint32_t nu = 0;
+        CHK(read_u32(d_tA + nl, &nu));
+        if (nu == 0) return 0;
+        k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
+        HIPC(hipGetLastError());
         for (int round = 0;; ++round) {
-            k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
             HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
-            PXD_DISPATCH_GS(launch_hook, nl);
+            PXD_DISPATCH_GS(launch_hook, d_cclist2, nu);
             HIPC(hipGetLastError());
             CHK(read_counters());
             if (!hc.hook_changed) break;
-            CHK(cc_flatten(nl));
+            CHK(cc_flatten(d_cclist2, nu));
             if (round > 100000) return set_err(PXD_ERR_STATE, "connected components did not converge");
         }
         return 0;

@@ -612,23 +612,23 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
     }
 }
 
-// Hooking rounds over the CC members.  For a same-part arc whose endpoints
-// carry different pointers a != b, slot max(a, b) is lowered to min(a, b) with
-// a plain store (racing stores keep every pointer below its node and inside
-// its component, so trees stay acyclic and a tree's root is its minimum node).
-// Sampling round: the first two arcs of every node.  Full rounds skip nodes
-// whose pointer equals their part's sampled giant root: every arc leaving the
-// giant tree has its other endpoint outside it and is hooked from that side.
-// A full round that finds no differing pointers on flat trees is final.
+// Connected components of every CC part (parent pointers `root`, members in
+// `list`).  Invariant of every kernel below: a pointer only ever moves to a
+// smaller node of the same component, so trees stay acyclic and the root of a
+// tree is its minimum node.
+//
+// Sampling: for the first `lim` same-part arcs of each node, the larger of the
+// two endpoint pointers is lowered to the smaller with a plain store (racing
+// stores may lose a link, never break the invariant).
 __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
                                                   const uint32_t* __restrict__ nbr,
                                                   const uint32_t* __restrict__ part,
-                                                  uint32_t* __restrict__ root) {
+                                                  uint32_t* __restrict__ root, uint32_t lim) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
-        const uint32_t start = row[u], d = min(dint[u], 2u);
+        const uint32_t start = row[u], d = min(dint[u], lim);
         const uint32_t pp = part[u];
         for (uint32_t j = 0; j < d; ++j) {
             const uint32_t v = nbr[start + j];
@@ -641,41 +641,64 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
     }
 }
 
+// Certificate after sampling and flattening: a part whose members all point to
+// one root is connected.  Parts with a member off the part's sampled root are
+// marked unresolved.
+__global__ void k_cc_check(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
+                           const uint32_t* __restrict__ giant, const uint32_t* __restrict__ root,
+                           uint32_t* __restrict__ unresolved) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        const uint32_t pp = part[u];
+        if (root[u] != giant[pp] && unresolved[pp] == 0u) unresolved[pp] = 1u;
+    }
+}
+
+__global__ void k_cc_flags(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
+                           const uint32_t* __restrict__ unresolved, uint32_t* __restrict__ f) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x)
+        f[i] = unresolved[part[list[i]]];
+}
+
+// Exact rounds on the members of unresolved parts, GS lanes per node:
+// m = smallest pointer among the node's same-part neighbours; if m is below
+// the node's own pointer pu, slot pu is lowered to m.  A round on flat trees
+// that finds no smaller neighbour pointer anywhere is final.
 template <int GS>
 __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ part,
-                                                const uint32_t* __restrict__ giant,
                                                 uint32_t* __restrict__ root, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const uint32_t groups = gridDim.x * (BS / GS);
     uint32_t changed = 0;
     for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
         const uint32_t u = list[i];
+        const uint32_t start = row[u], d = dint[u];
         const uint32_t pp = part[u];
         const uint32_t pu = root[u];
-        if (pu == giant[pp]) continue;                 // uniform within the group
-        const uint32_t start = row[u], d = dint[u];
+        uint32_t m = pu;
         for (uint32_t j = lane; j < d; j += GS) {
             const uint32_t v = nbr[start + j];
-            if (part[v] != pp) continue;
-            const uint32_t pv = root[v];
-            if (pv == pu) continue;
+            if (part[v] == pp) m = min(m, root[v]);
+        }
+        m = group_min<GS>(m);
+        if (lane == 0 && m < pu) {
             changed = 1;
-            const uint32_t hi = max(pu, pv), lo = min(pu, pv);
-            if (root[hi] > lo) root[hi] = lo;
+            if (root[pu] > m) atomicMin(&root[pu], m);
         }
     }
     if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->hook_changed, __ATOMIC_RELAXED) == 0u)
         atomicOr(&cnt->hook_changed, 1u);
 }
 
-// giant-root candidate of every part: the pointer of the part's middle member
+// sampled root of every part: the pointer of the part's middle member
 __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                            const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
-                           const uint32_t* __restrict__ root, uint32_t* __restrict__ giant) {
+                           const uint32_t* __restrict__ root, uint32_t* __restrict__ giant,
+                           uint32_t* __restrict__ unresolved) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const uint32_t f = cflag[c];
         if (!(f & CF_CC)) continue;
@@ -683,6 +706,8 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
         const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
         giant[c << 1] = o ? root[pnix[b + o / 2]] : INVALID;
         giant[(c << 1) | 1] = (len > o) ? root[pnix[b + o + (len - o) / 2]] : INVALID;
+        unresolved[c << 1] = 0u;
+        unresolved[(c << 1) | 1] = 0u;
     }
 }
 
