@@ -167,7 +167,11 @@ def main():
             d1 = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
             d1.initialize(g)
             t = time.perf_counter()
-            it1 = d1.expanderDecomposition()
+            try:
+                it1 = d1.expanderDecomposition()
+            except px.PxdError as err:
+                log(f"plain graph {i} (n={g.n}, m={g.m}, avg degree {2 * g.m / max(g.n, 1):.1f}) failed: {err}")
+                raise
             tot += time.perf_counter() - t
             ids1, k1 = d1.getFinalPartition()
             if a.workload == "c4" or len(graphs) > 1:

@@ -149,6 +149,17 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Device-coherent (agent scope) relaxed accesses for words that other threads
+// of the same launch may write.  Each XCD has its own L2; plain accesses to
+// such words are data races, these compile to sc1 accesses that every XCD
+// sees alike.
+__device__ __forceinline__ uint32_t ld_dev(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int G>
 __device__ __forceinline__ uint32_t group_min(uint32_t v) {
 #pragma unroll

@@ -675,6 +675,48 @@ struct pxd_engine {
         }
     }
 
+    // Non-convergence report: the state of the exact phase, read back.
+    int cc_diagnose(uint32_t nu, int rounds) {
+        std::vector<uint32_t> root(N), part(N), list(nu), row(N + 1), dint(N), nbr(A);
+        HIPC(hipStreamSynchronize(st));
+        HIPC(hipMemcpy(root.data(), d_root, N * 4ull, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(part.data(), d_part, N * 4ull, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(list.data(), d_cclist2, nu * 4ull, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(row.data(), d_row, (N + 1) * 4ull, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(dint.data(), d_dint, N * 4ull, hipMemcpyDeviceToHost));
+        if (A) HIPC(hipMemcpy(nbr.data(), d_nbr, A * 4ull, hipMemcpyDeviceToHost));
+        uint64_t roots = 0, notflat = 0, bad = 0, above = 0, outside = 0;
+        std::string ex;
+        std::vector<uint8_t> inl(N, 0);
+        for (uint32_t u : list) inl[u] = 1;
+        for (uint32_t u : list) {
+            const uint32_t pu = root[u];
+            if (pu == u) ++roots;
+            if (root[pu] != pu) ++notflat;
+            if (pu > u) ++above;
+            if (!inl[pu]) ++outside;
+            uint32_t m = pu, mv = 0;
+            for (uint32_t j = row[u]; j < row[u] + dint[u]; ++j) {
+                const uint32_t v = nbr[j];
+                if (part[v] == part[u] && root[v] < m) { m = root[v]; mv = v; }
+            }
+            if (m < pu) {
+                if (++bad <= 4) {
+                    char b[200];
+                    snprintf(b, sizeof b, " [u=%u pu=%u root[pu]=%u v=%u m=%u root[m]=%u inlist(v)=%d]", u, pu,
+                             root[pu], mv, m, root[m], (int)inl[mv]);
+                    ex += b;
+                }
+            }
+        }
+        return set_err(PXD_ERR_STATE,
+                       "connected components did not converge after %d rounds: %u members, %llu roots, %llu not "
+                       "flat, %llu pointers above their node, %llu pointing outside the list, %llu with a smaller "
+                       "neighbour pointer;%s",
+                       rounds, nu, (unsigned long long)roots, (unsigned long long)notflat, (unsigned long long)above,
+                       (unsigned long long)outside, (unsigned long long)bad, ex.c_str());
+    }
+
     // Components of every CC part (clusters flagged CF_CC in table T):
     // sampling over the first four arcs of each node, flattening, then the
     // one-root certificate; only members of parts that fail it go through the
@@ -702,7 +744,8 @@ struct pxd_engine {
             CHK(read_counters());
             if (!hc.hook_changed) break;
             CHK(cc_flatten(d_cclist2, nu));
-            if (round > 100000) return set_err(PXD_ERR_STATE, "connected components did not converge");
+            // unreachable while every changed round lowers a pointer; kept as a guard
+            if (round >= 100000) return cc_diagnose(nu, round);
         }
         return 0;
     }

@@ -617,9 +617,10 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
 // smaller node of the same component, so trees stay acyclic and the root of a
 // tree is its minimum node.
 //
-// Sampling: for the first `lim` same-part arcs of each node, the larger of the
-// two endpoint pointers is lowered to the smaller with a plain store (racing
-// stores may lose a link, never break the invariant).
+// Sampling: for the first `lim` same-part arcs of each node, the slot of the
+// larger of the two endpoint pointers is lowered to the smaller (atomicMin).
+// Every access to `root` in these kernels is device-coherent: threads on
+// different XCDs read and write the same slots within one launch.
 __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
@@ -633,10 +634,10 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
         for (uint32_t j = 0; j < d; ++j) {
             const uint32_t v = nbr[start + j];
             if (part[v] != pp) continue;
-            const uint32_t a = root[u], b = root[v];
+            const uint32_t a = ld_dev(&root[u]), b = ld_dev(&root[v]);
             if (a == b) continue;
             const uint32_t hi = max(a, b), lo = min(a, b);
-            if (root[hi] > lo) root[hi] = lo;
+            if (ld_dev(&root[hi]) > lo) atomicMin(&root[hi], lo);
         }
     }
 }
@@ -645,12 +646,12 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
 // one root is connected.  Parts with a member off the part's sampled root are
 // marked unresolved.
 __global__ void k_cc_check(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
-                           const uint32_t* __restrict__ giant, const uint32_t* __restrict__ root,
+                           const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
                            uint32_t* __restrict__ unresolved) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         const uint32_t pp = part[u];
-        if (root[u] != giant[pp] && unresolved[pp] == 0u) unresolved[pp] = 1u;
+        if (ld_dev(&root[u]) != giant[pp] && unresolved[pp] == 0u) atomicOr(&unresolved[pp], 1u);
     }
 }
 
@@ -662,8 +663,12 @@ __global__ void k_cc_flags(uint32_t nl, const uint32_t* __restrict__ list, const
 
 // Exact rounds on the members of unresolved parts, GS lanes per node:
 // m = smallest pointer among the node's same-part neighbours; if m is below
-// the node's own pointer pu, slot pu is lowered to m.  A round on flat trees
-// that finds no smaller neighbour pointer anywhere is final.
+// the node's own pointer pu, slots pu and u are both lowered to m.  A round
+// counts as changed only when one of its atomics lowered a slot, so the sum
+// of pointers falls in every changed round and the loop ends.  In a round
+// with no change every pointer is constant, so every node's pointer is at
+// most each same-part neighbour's: all nodes of a component share one
+// pointer, which (being a node of the component) points to itself.
 template <int GS>
 __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,
@@ -678,16 +683,16 @@ __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __r
         const uint32_t u = list[i];
         const uint32_t start = row[u], d = dint[u];
         const uint32_t pp = part[u];
-        const uint32_t pu = root[u];
+        const uint32_t pu = ld_dev(&root[u]);
         uint32_t m = pu;
         for (uint32_t j = lane; j < d; j += GS) {
             const uint32_t v = nbr[start + j];
-            if (part[v] == pp) m = min(m, root[v]);
+            if (part[v] == pp) m = min(m, ld_dev(&root[v]));
         }
         m = group_min<GS>(m);
         if (lane == 0 && m < pu) {
-            changed = 1;
-            if (root[pu] > m) atomicMin(&root[pu], m);
+            if (ld_dev(&root[pu]) > m && atomicMin(&root[pu], m) > m) changed = 1;
+            if (atomicMin(&root[u], m) > m) changed = 1;
         }
     }
     if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->hook_changed, __ATOMIC_RELAXED) == 0u)
@@ -697,15 +702,15 @@ __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __r
 // sampled root of every part: the pointer of the part's middle member
 __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                            const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
-                           const uint32_t* __restrict__ root, uint32_t* __restrict__ giant,
+                           uint32_t* __restrict__ root, uint32_t* __restrict__ giant,
                            uint32_t* __restrict__ unresolved) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const uint32_t f = cflag[c];
         if (!(f & CF_CC)) continue;
         const uint32_t b = cs[c], len = cs[c + 1] - b;
         const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
-        giant[c << 1] = o ? root[pnix[b + o / 2]] : INVALID;
-        giant[(c << 1) | 1] = (len > o) ? root[pnix[b + o + (len - o) / 2]] : INVALID;
+        giant[c << 1] = o ? ld_dev(&root[pnix[b + o / 2]]) : INVALID;
+        giant[(c << 1) | 1] = (len > o) ? ld_dev(&root[pnix[b + o + (len - o) / 2]]) : INVALID;
         unresolved[c << 1] = 0u;
         unresolved[(c << 1) | 1] = 0u;
     }
@@ -719,14 +724,14 @@ __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __r
         const uint32_t i = i0 + threadIdx.x;
         if (i >= nl) continue;
         const uint32_t u = list[i];
-        const uint32_t r0 = root[u];
+        const uint32_t r0 = ld_dev(&root[u]);
         uint32_t r = r0;
         for (int s = 0; s < 8; ++s) {
-            const uint32_t rr = root[r];
+            const uint32_t rr = ld_dev(&root[r]);
             if (rr == r) break;
             r = rr;
         }
-        if (r != r0) { root[u] = r; changed = 1; }
+        if (r != r0) { st_dev(&root[u], r); changed = 1; }
     }
     if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->jump_changed, __ATOMIC_RELAXED) == 0u)
         atomicOr(&cnt->jump_changed, 1u);

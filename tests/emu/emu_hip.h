@@ -119,6 +119,9 @@ template <class T> inline T atomicMin(T* p, T v) {
     while (v < old && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
     return old;
 }
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <class T> inline T __hip_atomic_load(T* p, int order, int) { (void)order; return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+template <class T> inline void __hip_atomic_store(T* p, T v, int order, int) { (void)order; __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicMax(T* p, T v) {
     T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
     while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
