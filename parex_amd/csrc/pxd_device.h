@@ -44,30 +44,26 @@ __device__ __forceinline__ void fix_add128(Fix128& a, uint64_t blo, uint64_t bhi
     a.lo = lo;
 }
 
-// Adds trunc(y * 2^72) to the accumulator.  Terms with |y| >= 2^31 set `ovf`.
+// Adds trunc(y * 2^72) (truncation toward zero) to the accumulator, without
+// branches.  |y| * 2^72 = M * 2^sh with the 53-bit significand M: for sh < 0
+// the magnitude is M >> -sh (truncated; zero and subnormal y shift out
+// entirely), the sign is applied to that 64-bit value, and for sh > 0 the
+// signed value is shifted left across the two words, which is exact.  Terms
+// with |y| >= 2^31 (sh > 50, also inf/nan) set `ovf` and leave the sum
+// meaningless.
 __device__ __forceinline__ void fix_add_term(Fix128& a, double y, uint32_t& ovf) {
-    uint64_t b = (uint64_t)__double_as_longlong(y);
-    uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
-    if (E == 0) return;
-    uint64_t M = (b & ((1ull << 52) - 1)) | (1ull << 52);
-    int sh = (int)E - (1075 - FIX_SHIFT);
-    uint64_t qlo, qhi;
-    if (sh >= 0) {
-        if (sh > 50) { ovf = 1; return; }
-        qlo = M << sh;
-        qhi = sh ? (M >> (64 - sh)) : 0ull;
-    } else {
-        int r = -sh;
-        qlo = (r >= 53) ? 0ull : (M >> r);
-        qhi = 0ull;
-    }
-    if (b >> 63) {          // subtract: add two's complement of (qhi, qlo)
-        uint64_t nlo = ~qlo + 1ull;
-        uint64_t nhi = ~qhi + (nlo == 0ull ? 1ull : 0ull);
-        fix_add128(a, nlo, nhi);
-    } else {
-        fix_add128(a, qlo, qhi);
-    }
+    const uint64_t b = (uint64_t)__double_as_longlong(y);
+    const int E = (int)((b >> 52) & 0x7FFu);
+    const uint64_t M = (b & ((1ull << 52) - 1)) | (1ull << 52);
+    const int sh = E - (1075 - FIX_SHIFT);
+    ovf |= (uint32_t)(sh > 50);
+    const uint32_t l = (uint32_t)min(max(sh, 0), 63);
+    const uint32_t r = (uint32_t)min(max(-sh, 0), 63);
+    const int64_t m = (int64_t)b >> 63;                            // 0 or -1
+    const int64_t t = (int64_t)(((M >> r) ^ (uint64_t)m) - (uint64_t)m);   // signed, |t| < 2^53
+    const uint64_t qlo = (uint64_t)t << l;
+    const uint64_t qhi = (uint64_t)((t >> 1) >> (63 - l));         // t >> (64 - l), sign-filled
+    fix_add128(a, qlo, qhi);
 }
 
 // Correctly rounded conversion of a value below 2^63 (exact halves, one rounding).
@@ -148,6 +144,24 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// XCD-aware split of a grid-stride loop over n items.  Workgroups are
+// dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8, each XCD
+// with its own L2), so XCD x takes the contiguous x-th eighth of the items and
+// its blocks stride through that eighth: neighbouring items, whose gathers
+// touch the same lines, share one L2.  The grid must be a multiple of 8; every
+// item is visited exactly once.  `per` = items a block covers per stride,
+// `local` = this thread's item within the block's share.
+constexpr uint32_t NXCD = 8;
+struct XcdSpan {
+    uint32_t first, end, stride;
+};
+__device__ __forceinline__ XcdSpan xcd_span(uint32_t n, uint32_t per, uint32_t local) {
+    const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
+    const uint32_t chunk = n / NXCD + (n % NXCD != 0);
+    const uint32_t b = min(n, x * chunk), e = min(n, b + chunk);
+    return XcdSpan{b + lb * per + local, e, nb * per};
+}
 
 // Device-coherent (agent scope) relaxed accesses for words that other threads
 // of the same launch may write.  Each XCD has its own L2; plain accesses to

@@ -57,6 +57,12 @@ static inline uint32_t grid_for(uint64_t n, uint32_t per_block) {
     return (uint32_t)g;
 }
 
+// grid for kernels that split their items per XCD (xcd_span): a multiple of 8
+static inline uint32_t grid_xcd(uint64_t n, uint32_t per_block) {
+    const uint32_t g = grid_for(n, per_block);
+    return (g + NXCD - 1) / NXCD * NXCD;
+}
+
 struct ClusterTable {
     uint32_t *cs = nullptr, *cgraph = nullptr, *cmin = nullptr, *cflag = nullptr, *cvol = nullptr;
     int32_t* clabel = nullptr;
@@ -484,22 +490,22 @@ struct pxd_engine {
     template <int GSZ>
     void launch_walk() {
         const uint32_t groups_per_block = BS / GSZ;
-        k_walk<GSZ><<<grid_for(nalist, groups_per_block), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
+        k_walk<GSZ><<<grid_xcd(nalist, groups_per_block), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
                                                                          d_y[cur], d_x[cur], d_x[cur ^ 1],
                                                                          d_y[cur ^ 1], d_xkey, d_cnt);
     }
     template <int GSZ>
     void launch_contrib() {
-        k_contrib<GSZ><<<grid_for(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
+        k_contrib<GSZ><<<grid_xcd(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
                                                                      d_xkey, d_ediff);
     }
     template <int GSZ>
     void launch_hook(const uint32_t* list, uint32_t nl) {
-        k_cc_hook<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
+        k_cc_hook<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
-        k_disable<GSZ><<<grid_for(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
+        k_disable<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
                                                                 d_x[cur], d_y[cur]);
     }
 
@@ -723,7 +729,7 @@ struct pxd_engine {
     // exact hooking rounds.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
-        k_cc_sample<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
+        k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
         HIPC(hipGetLastError());
         CHK(cc_flatten(d_cclist, nl));
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant,

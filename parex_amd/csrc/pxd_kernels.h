@@ -106,25 +106,27 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
                                              double* __restrict__ yn, uint32_t* __restrict__ xkey,
                                              Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
-    const uint32_t groups = gridDim.x * (BS / GS);
+    const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
     uint32_t ovf = 0;
-    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < na; i += groups) {
+    for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
         Fix128 acc = {0ull, 0ull};
-        // four independent index loads, then four value gathers in flight per lane
+        // four independent index loads, then four value gathers in flight per
+        // lane; slots past the row end re-read the row's last arc (no branch,
+        // same line) and add +0.0
+        const uint32_t* p = nbr + start;
         for (uint32_t j = lane; j < d; j += 4 * GS) {
-            const uint32_t* p = nbr + start;
             const bool b1 = j + GS < d, b2 = j + 2 * GS < d, b3 = j + 3 * GS < d;
             const uint32_t v0 = p[j];
-            const uint32_t v1 = b1 ? p[j + GS] : v0;
-            const uint32_t v2 = b2 ? p[j + 2 * GS] : v0;
-            const uint32_t v3 = b3 ? p[j + 3 * GS] : v0;
+            const uint32_t v1 = p[min(j + GS, d - 1)];
+            const uint32_t v2 = p[min(j + 2 * GS, d - 1)];
+            const uint32_t v3 = p[min(j + 3 * GS, d - 1)];
             const double y0 = y[v0], y1 = y[v1], y2 = y[v2], y3 = y[v3];
             fix_add_term(acc, y0, ovf);
-            if (b1) fix_add_term(acc, y1, ovf);
-            if (b2) fix_add_term(acc, y2, ovf);
-            if (b3) fix_add_term(acc, y3, ovf);
+            fix_add_term(acc, b1 ? y1 : 0.0, ovf);
+            fix_add_term(acc, b2 ? y2 : 0.0, ovf);
+            fix_add_term(acc, b3 ? y3 : 0.0, ovf);
         }
         acc = group_sum_fix<GS>(acc);
         if (lane == 0) {
@@ -152,20 +154,20 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
                                                 const uint32_t* __restrict__ xkey,
                                                 int32_t* __restrict__ ediff) {
     const uint32_t lane = threadIdx.x & (GS - 1);
-    const uint32_t groups = gridDim.x * (BS / GS);
-    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < na; i += groups) {
+    const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
+    for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
         const uint32_t mk = xkey[u];
         int32_t c = 0;
+        const uint32_t* p = nbr + start;
         for (uint32_t j = lane; j < d; j += 4 * GS) {
-            const uint32_t* p = nbr + start;
             uint32_t v[4], k[4];
             bool b[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 4; ++q) {           // clamped: no branch around the loads
                 b[q] = j + q * GS < d;
-                v[q] = b[q] ? p[j + q * GS] : u;
+                v[q] = p[min(j + q * GS, d - 1)];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) k[q] = xkey[v[q]];
@@ -627,7 +629,8 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
                                                   const uint32_t* __restrict__ nbr,
                                                   const uint32_t* __restrict__ part,
                                                   uint32_t* __restrict__ root, uint32_t lim) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+    const XcdSpan sp = xcd_span(nl, BS, threadIdx.x);
+    for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
         const uint32_t start = row[u], d = min(dint[u], lim);
         const uint32_t pp = part[u];
@@ -677,9 +680,9 @@ __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ part,
                                                 uint32_t* __restrict__ root, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
-    const uint32_t groups = gridDim.x * (BS / GS);
+    const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
     uint32_t changed = 0;
-    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
+    for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
         const uint32_t start = row[u], d = dint[u];
         const uint32_t pp = part[u];
@@ -985,8 +988,8 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ root,
                                                 const double* __restrict__ x, double* __restrict__ y) {
     const uint32_t lane = threadIdx.x & (GS - 1);
-    const uint32_t groups = gridDim.x * (BS / GS);
-    for (uint32_t i = blockIdx.x * (BS / GS) + threadIdx.x / GS; i < nl; i += groups) {
+    const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
+    for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
         if (!alive[u]) continue;                      // uniform within the group
         const uint32_t start = row[u], d = dint[u];
