@@ -125,7 +125,6 @@ struct pxd_engine {
     uint32_t* d_oneseg = nullptr;
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
-    uint32_t* d_unres = nullptr;
     uint32_t* d_cclist2 = nullptr;
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
@@ -324,7 +323,6 @@ struct pxd_engine {
         CHK(alloc(&d_oneseg, 2));
         CHK(alloc(&d_gclu, G));
         CHK(alloc(&d_giant, 2 * n1 + 2));
-        CHK(alloc(&d_unres, 2 * n1 + 2));
         CHK(alloc(&d_cclist2, N));
         CHK(alloc(&d_u64, 1));
         CHK(alloc(&d_cnt, 1));
@@ -500,8 +498,8 @@ struct pxd_engine {
                                                                      d_xkey, d_ediff);
     }
     template <int GSZ>
-    void launch_hook(const uint32_t* list, uint32_t nl) {
-        k_cc_hook<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
+    void launch_link(const uint32_t* list, uint32_t nl) {
+        k_cc_link<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
@@ -681,61 +679,17 @@ struct pxd_engine {
         }
     }
 
-    // Non-convergence report: the state of the exact phase, read back.
-    int cc_diagnose(uint32_t nu, int rounds) {
-        std::vector<uint32_t> root(N), part(N), list(nu), row(N + 1), dint(N), nbr(A);
-        HIPC(hipStreamSynchronize(st));
-        HIPC(hipMemcpy(root.data(), d_root, N * 4ull, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(part.data(), d_part, N * 4ull, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(list.data(), d_cclist2, nu * 4ull, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(row.data(), d_row, (N + 1) * 4ull, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(dint.data(), d_dint, N * 4ull, hipMemcpyDeviceToHost));
-        if (A) HIPC(hipMemcpy(nbr.data(), d_nbr, A * 4ull, hipMemcpyDeviceToHost));
-        uint64_t roots = 0, notflat = 0, bad = 0, above = 0, outside = 0;
-        std::string ex;
-        std::vector<uint8_t> inl(N, 0);
-        for (uint32_t u : list) inl[u] = 1;
-        for (uint32_t u : list) {
-            const uint32_t pu = root[u];
-            if (pu == u) ++roots;
-            if (root[pu] != pu) ++notflat;
-            if (pu > u) ++above;
-            if (!inl[pu]) ++outside;
-            uint32_t m = pu, mv = 0;
-            for (uint32_t j = row[u]; j < row[u] + dint[u]; ++j) {
-                const uint32_t v = nbr[j];
-                if (part[v] == part[u] && root[v] < m) { m = root[v]; mv = v; }
-            }
-            if (m < pu) {
-                if (++bad <= 4) {
-                    char b[200];
-                    snprintf(b, sizeof b, " [u=%u pu=%u root[pu]=%u v=%u m=%u root[m]=%u inlist(v)=%d]", u, pu,
-                             root[pu], mv, m, root[m], (int)inl[mv]);
-                    ex += b;
-                }
-            }
-        }
-        return set_err(PXD_ERR_STATE,
-                       "connected components did not converge after %d rounds: %u members, %llu roots, %llu not "
-                       "flat, %llu pointers above their node, %llu pointing outside the list, %llu with a smaller "
-                       "neighbour pointer;%s",
-                       rounds, nu, (unsigned long long)roots, (unsigned long long)notflat, (unsigned long long)above,
-                       (unsigned long long)outside, (unsigned long long)bad, ex.c_str());
-    }
-
     // Components of every CC part (clusters flagged CF_CC in table T):
-    // sampling over the first four arcs of each node, flattening, then the
-    // one-root certificate; only members of parts that fail it go through the
-    // exact hooking rounds.
+    // sampling over the first four arcs of each node and flattening, then
+    // exact linking rounds over the members outside their part's giant tree,
+    // then a final flattening of all members.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
         k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
         HIPC(hipGetLastError());
         CHK(cc_flatten(d_cclist, nl));
-        k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant,
-                                                    d_unres);
-        k_cc_check<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_unres);
-        k_cc_flags<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_unres, d_tC);
+        k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
+        k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC);
         HIPC(hipGetLastError());
         CHK(scan(d_tC, d_tA, nl));
         uint32_t nu = 0;
@@ -744,16 +698,18 @@ struct pxd_engine {
         k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
         HIPC(hipGetLastError());
         for (int round = 0;; ++round) {
-            HIPC(hipMemsetAsync(&d_cnt->hook_changed, 0, 4, st));
-            PXD_DISPATCH_GS(launch_hook, d_cclist2, nu);
+            HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
+            PXD_DISPATCH_GS(launch_link, d_cclist2, nu);
             HIPC(hipGetLastError());
             CHK(read_counters());
-            if (!hc.hook_changed) break;
+            if (!hc.link_changed) break;
             CHK(cc_flatten(d_cclist2, nu));
-            // unreachable while every changed round lowers a pointer; kept as a guard
-            if (round >= 100000) return cc_diagnose(nu, round);
+            // every changed round lowers a pointer, so this bound is never reached
+            if (round >= 100000)
+                return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nu);
         }
-        return 0;
+        // a giant root may have been linked below another root
+        return cc_flatten(d_cclist, nl);
     }
 
     int run(int32_t* iterations) {

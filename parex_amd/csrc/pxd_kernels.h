@@ -32,7 +32,7 @@ struct Counters {
     uint32_t nwave, nblock, nlarge;
     uint32_t ncut, ncc;
     uint32_t cclist_n;
-    uint32_t hook_changed, jump_changed;
+    uint32_t link_changed, jump_changed;
     uint32_t pad[7];
 };
 
@@ -617,12 +617,14 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
 // Connected components of every CC part (parent pointers `root`, members in
 // `list`).  Invariant of every kernel below: a pointer only ever moves to a
 // smaller node of the same component, so trees stay acyclic and the root of a
-// tree is its minimum node.
+// tree is its minimum node.  Every write to `root` is atomic (or, in
+// k_cc_jump, to the thread's own slot): threads on different XCDs write the
+// same slots within one launch.
 //
 // Sampling: for the first `lim` same-part arcs of each node, the slot of the
 // larger of the two endpoint pointers is lowered to the smaller (atomicMin).
-// Every access to `root` in these kernels is device-coherent: threads on
-// different XCDs read and write the same slots within one launch.
+// Plain reads may be stale; a stale pointer is still a smaller node of the
+// same component, so it only weakens a link.
 __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
@@ -637,43 +639,58 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
         for (uint32_t j = 0; j < d; ++j) {
             const uint32_t v = nbr[start + j];
             if (part[v] != pp) continue;
-            const uint32_t a = ld_dev(&root[u]), b = ld_dev(&root[v]);
+            const uint32_t a = root[u], b = root[v];
             if (a == b) continue;
             const uint32_t hi = max(a, b), lo = min(a, b);
-            if (ld_dev(&root[hi]) > lo) atomicMin(&root[hi], lo);
+            if (root[hi] > lo) atomicMin(&root[hi], lo);
         }
     }
 }
 
-// Certificate after sampling and flattening: a part whose members all point to
-// one root is connected.  Parts with a member off the part's sampled root are
-// marked unresolved.
-__global__ void k_cc_check(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
-                           const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
-                           uint32_t* __restrict__ unresolved) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
-        const uint32_t u = list[i];
-        const uint32_t pp = part[u];
-        if (ld_dev(&root[u]) != giant[pp] && unresolved[pp] == 0u) atomicOr(&unresolved[pp], 1u);
+// Giant root of every part after sampling and flattening: the pointer of the
+// part's middle member (by position).  Members already in that tree skip the
+// exact phase.
+__global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
+                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
+                           uint32_t* __restrict__ root, uint32_t* __restrict__ giant) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t f = cflag[c];
+        if (!(f & CF_CC)) continue;
+        const uint32_t b = cs[c], len = cs[c + 1] - b;
+        const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
+        giant[c << 1] = o ? ld_dev(&root[pnix[b + o / 2]]) : INVALID;
+        giant[(c << 1) | 1] = (len > o) ? ld_dev(&root[pnix[b + o + (len - o) / 2]]) : INVALID;
     }
 }
 
-__global__ void k_cc_flags(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
-                           const uint32_t* __restrict__ unresolved, uint32_t* __restrict__ f) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x)
-        f[i] = unresolved[part[list[i]]];
+// f[i] = 1 for members outside their part's giant tree
+__global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
+                              const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
+                              uint32_t* __restrict__ f) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        f[i] = ld_dev(&root[u]) != giant[part[u]] ? 1u : 0u;
+    }
 }
 
-// Exact rounds on the members of unresolved parts, GS lanes per node:
-// m = smallest pointer among the node's same-part neighbours; if m is below
-// the node's own pointer pu, slots pu and u are both lowered to m.  A round
-// counts as changed only when one of its atomics lowered a slot, so the sum
-// of pointers falls in every changed round and the loop ends.  In a round
-// with no change every pointer is constant, so every node's pointer is at
-// most each same-part neighbour's: all nodes of a component share one
-// pointer, which (being a node of the component) points to itself.
+__device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
+    for (;;) {                        // strictly decreasing: ends at a root
+        const uint32_t p = ld_dev(&root[r]);
+        if (p == r) return r;
+        r = p;
+    }
+}
+
+// Exact rounds over the members outside the giant trees, GS lanes per node:
+// for every same-part arc (u, v) with find(u) != find(v) the larger root's
+// slot is lowered to the smaller root.  A round counts as changed only when an
+// atomic lowered a slot, so the sum of pointers falls in every changed round
+// and the loop ends.  In a round without change every pointer is constant and
+// every processed arc joins two nodes with the same root (else the larger root
+// would have been lowered); arcs between giant members lie inside the giant's
+// flat tree, and an arc with one giant end is processed from its other end.
 template <int GS>
-__global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
@@ -686,37 +703,21 @@ __global__ void __launch_bounds__(BS) k_cc_hook(uint32_t nl, const uint32_t* __r
         const uint32_t u = list[i];
         const uint32_t start = row[u], d = dint[u];
         const uint32_t pp = part[u];
-        const uint32_t pu = ld_dev(&root[u]);
-        uint32_t m = pu;
+        uint32_t ru = cc_find(root, u);
         for (uint32_t j = lane; j < d; j += GS) {
             const uint32_t v = nbr[start + j];
-            if (part[v] == pp) m = min(m, ld_dev(&root[v]));
-        }
-        m = group_min<GS>(m);
-        if (lane == 0 && m < pu) {
-            if (ld_dev(&root[pu]) > m && atomicMin(&root[pu], m) > m) changed = 1;
-            if (atomicMin(&root[u], m) > m) changed = 1;
+            if (part[v] != pp) continue;
+            const uint32_t rv = cc_find(root, v);
+            if (rv == ru) continue;
+            const uint32_t hi = max(ru, rv), lo = min(ru, rv);
+            if (ld_dev(&root[hi]) > lo && atomicMin(&root[hi], lo) > lo) {
+                changed = 1;
+                if (hi == ru) ru = lo;    // u's tree now hangs below lo
+            }
         }
     }
-    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->hook_changed, __ATOMIC_RELAXED) == 0u)
-        atomicOr(&cnt->hook_changed, 1u);
-}
-
-// sampled root of every part: the pointer of the part's middle member
-__global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
-                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
-                           uint32_t* __restrict__ root, uint32_t* __restrict__ giant,
-                           uint32_t* __restrict__ unresolved) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const uint32_t f = cflag[c];
-        if (!(f & CF_CC)) continue;
-        const uint32_t b = cs[c], len = cs[c + 1] - b;
-        const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
-        giant[c << 1] = o ? ld_dev(&root[pnix[b + o / 2]]) : INVALID;
-        giant[(c << 1) | 1] = (len > o) ? ld_dev(&root[pnix[b + o + (len - o) / 2]]) : INVALID;
-        unresolved[c << 1] = 0u;
-        unresolved[(c << 1) | 1] = 0u;
-    }
+    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->link_changed, __ATOMIC_RELAXED) == 0u)
+        atomicOr(&cnt->link_changed, 1u);
 }
 
 // pointer jumping, up to 8 steps per node and launch
