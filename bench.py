@@ -61,6 +61,18 @@ def workload(name, graphs_limit=None):
     raise SystemExit(f"unknown workload {name}")
 
 
+def profiled_traffic(workload):
+    """HBM bytes per k_walk launch from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"k_walk_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(graphs, budget_s):
     """Reference single-threaded CPU decomposition (oracle/_ref) on a bounded
     prefix of the workload; falls back to the oracle's restatement."""
@@ -125,7 +137,8 @@ def main():
     dm = px.DeviceManager(px.Params(keep_original=True, record_node_data=False))
     t = time.perf_counter()
     dm.initialize_batch(graphs)
-    log(f"batched initialize {time.perf_counter() - t:.2f} s")
+    init_s = time.perf_counter() - t
+    log(f"batched initialize {init_s:.2f} s")
     for w in range(a.warmup):
         dm.reset()
         it = dm.expanderDecomposition()
@@ -150,7 +163,8 @@ def main():
     dm.enableWalkTiming(False)
     achieved = walk_bytes / (walk_ms / 1e3) / 1e9 if walk_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(a.workload) if a.graphs is None else None,
                 "kernel": "k_walk", "launches": launches,
                 "avg_launch_us": round(walk_ms * 1e3 / max(launches, 1), 2),
                 "alg_bytes_per_launch": round(walk_bytes / max(launches, 1))}
@@ -189,6 +203,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic", "config": cfg,
+        "initialize_s": round(init_s, 3),
+        "value_incl_upload": m_tot / (ms / 1e3 + init_s),
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
         "batched_equals_plain": same,
         "iterations_max": int(batch_it.max()) if len(batch_it) else 0,
