@@ -680,35 +680,34 @@ struct pxd_engine {
     }
 
     // Components of every CC part (clusters flagged CF_CC in table T):
-    // sampling over the first four arcs of each node and flattening, then
-    // exact linking rounds over the members outside their part's giant tree,
-    // then a final flattening of all members.
+    // sampling over the first four arcs of each node, two pointer-jumping
+    // launches (trees need not end flat: roots are found by following
+    // pointers), exact linking rounds over the members outside their part's
+    // giant tree, then flattening of all members until no pointer moves.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
         k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
-        HIPC(hipGetLastError());
-        CHK(cc_flatten(d_cclist, nl));
+        for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
         k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC);
         HIPC(hipGetLastError());
         CHK(scan(d_tC, d_tA, nl));
         uint32_t nu = 0;
         CHK(read_u32(d_tA + nl, &nu));
-        if (nu == 0) return 0;
-        k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
-        HIPC(hipGetLastError());
-        for (int round = 0;; ++round) {
-            HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
-            PXD_DISPATCH_GS(launch_link, d_cclist2, nu);
+        if (nu) {
+            k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
             HIPC(hipGetLastError());
-            CHK(read_counters());
-            if (!hc.link_changed) break;
-            CHK(cc_flatten(d_cclist2, nu));
-            // every changed round lowers a pointer, so this bound is never reached
-            if (round >= 100000)
-                return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nu);
+            for (int round = 0;; ++round) {
+                HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
+                PXD_DISPATCH_GS(launch_link, d_cclist2, nu);
+                HIPC(hipGetLastError());
+                CHK(read_counters());
+                if (!hc.link_changed) break;
+                // every changed round lowers a pointer, so this bound is never reached
+                if (round >= 100000)
+                    return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nu);
+            }
         }
-        // a giant root may have been linked below another root
         return cc_flatten(d_cclist, nl);
     }
 

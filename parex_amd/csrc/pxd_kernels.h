@@ -647,9 +647,16 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
     }
 }
 
-// Giant root of every part after sampling and flattening: the pointer of the
-// part's middle member (by position).  Members already in that tree skip the
-// exact phase.
+__device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
+    for (;;) {                        // strictly decreasing: ends at a root
+        const uint32_t p = ld_dev(&root[r]);
+        if (p == r) return r;
+        r = p;
+    }
+}
+
+// Giant root of every part after sampling: the root of the part's middle
+// member (by position).  Members already in that tree skip the exact phase.
 __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                            const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
                            uint32_t* __restrict__ root, uint32_t* __restrict__ giant) {
@@ -658,8 +665,8 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
         if (!(f & CF_CC)) continue;
         const uint32_t b = cs[c], len = cs[c + 1] - b;
         const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
-        giant[c << 1] = o ? ld_dev(&root[pnix[b + o / 2]]) : INVALID;
-        giant[(c << 1) | 1] = (len > o) ? ld_dev(&root[pnix[b + o + (len - o) / 2]]) : INVALID;
+        giant[c << 1] = o ? cc_find(root, pnix[b + o / 2]) : INVALID;
+        giant[(c << 1) | 1] = (len > o) ? cc_find(root, pnix[b + o + (len - o) / 2]) : INVALID;
     }
 }
 
@@ -669,15 +676,7 @@ __global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, co
                               uint32_t* __restrict__ f) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
-        f[i] = ld_dev(&root[u]) != giant[part[u]] ? 1u : 0u;
-    }
-}
-
-__device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
-    for (;;) {                        // strictly decreasing: ends at a root
-        const uint32_t p = ld_dev(&root[r]);
-        if (p == r) return r;
-        r = p;
+        f[i] = cc_find(root, u) != giant[part[u]] ? 1u : 0u;
     }
 }
 
@@ -688,7 +687,8 @@ __device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
 // and the loop ends.  In a round without change every pointer is constant and
 // every processed arc joins two nodes with the same root (else the larger root
 // would have been lowered); arcs between giant members lie inside the giant's
-// flat tree, and an arc with one giant end is processed from its other end.
+// tree, and an arc with one giant end is processed from its other end.  Trees
+// need not be flat: every root is found by following pointers.
 template <int GS>
 __global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ row,

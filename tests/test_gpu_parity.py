@@ -23,7 +23,20 @@ def graph_cases():
         "islands": lambda: px.from_edges(40, [0, 1, 2, 5, 6, 7, 20, 21, 22, 22],
                                          [1, 2, 0, 6, 7, 5, 21, 22, 23, 24]),
         "sbm_small_seed9": lambda: px.sbm(600, 5, 0.08, 0.004, 9),
+        # mean degree 39 / 82 / 229: 8, 16 and 64 lanes per node
+        "sbm_dense_deg39": lambda: px.sbm(800, 4, 0.17, 0.01, 3),
+        "sbm_dense_deg82": lambda: px.sbm(1000, 4, 0.3, 0.01, 5),
+        "sbm_dense_deg229": lambda: px.sbm(500, 2, 0.9, 0.02, 7),
+        # one row of 2999 arcs on a ring: long rows under 4 lanes per node
+        "hub3000": lambda: hub(3000),
     }
+
+
+def hub(n):
+    """Node 0 joined to every other node, which also form a ring."""
+    fx = [0] * (n - 1) + list(range(1, n))
+    tx = list(range(1, n)) + [i % (n - 1) + 1 for i in range(2, n + 1)]
+    return px.from_edges(n, fx, tx)
 
 
 def _engine(g, **kw):
@@ -128,6 +141,59 @@ def test_end_to_end(name):
     assert k == ok and np.array_equal(ids, oids)
     cut, vol, ivol, cond = px.verify_partition(g, ids, k)
     assert vol.sum() == 2 * g.m and ivol.sum() + 2 * cut == 2 * g.m
+
+
+def test_end_to_end_sbm_50k():
+    """50,000 nodes, 20 blocks: clusters far above one workgroup's sort tile."""
+    g = px.sbm(50000, 20, 16 / 2499, 1 / 47500, 21)
+    dm = _engine(g)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    assert dm.expanderDecomposition() == ox.run()
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids)
+
+
+def test_batch_mixed_density_equals_solo():
+    """A batch picks its lanes per node from the batch's mean degree, a solo
+    run from its own: the exact walk sum makes both give the same bits."""
+    graphs = [px.sbm(500, 2, 0.9, 0.02, 7), px.sbm(600, 5, 0.08, 0.004, 9), hub(500),
+              px.sbm(1000, 4, 0.3, 0.01, 5)]
+    dm = px.DeviceManager()
+    dm.initialize_batch(graphs)
+    dm.expanderDecomposition()
+    ids, ks = dm.getFinalPartition()
+    its = dm.graphIterations()
+    off = 0
+    for i, g in enumerate(graphs):
+        solo = _engine(g)
+        assert its[i] == solo.expanderDecomposition()
+        sids, sk = solo.getFinalPartition()
+        assert ks[i] == sk and np.array_equal(ids[off:off + g.n], sids), f"graph {i}: partition differs"
+        off += g.n
+
+
+def test_cli_matches_oracle(tmp_path):
+    """bin/parex prints the reference's lines (main.cpp:15-64) for the engine's result."""
+    import os
+    import subprocess
+    g = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
+    path = str(tmp_path / "c1.pxg")
+    px.write_pxg(path, g)
+    cli = os.path.join(os.path.dirname(px.__file__), "bin", "parex")
+    r = subprocess.run([cli, path, "6582", "0.3", "1e-4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = dict(l.split(": ", 1) for l in r.stdout.splitlines() if ": " in l)
+    assert out["sparsity"] == "0.300000" and out["threshold"] == "0.000100"
+    assert "Begin Expander Decomposition" in r.stdout
+    dm = _engine(g)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    it = ox.run()
+    oids, ok = ox.final_partition()
+    cut, _, _, _ = px.verify_partition(g, oids, ok)
+    assert int(out["iterations"]) == it
+    assert (int(out["numClusters"]), int(out["cutEdges"])) == (ok, cut)
+    assert (int(out["nodes"]), int(out["edges"])) == (g.n, g.m)
 
 
 def test_batch_equals_solo():
