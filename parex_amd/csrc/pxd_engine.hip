@@ -125,6 +125,7 @@ struct pxd_engine {
     uint32_t* d_oneseg = nullptr;
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
+    uint32_t* d_nclu = nullptr;       // cluster of every active node (apply pass)
     uint32_t* d_cclist2 = nullptr;
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
@@ -266,6 +267,7 @@ struct pxd_engine {
         CHK(alloc(&d_ediff, N));
         CHK(alloc(&d_root, N));
         CHK(alloc(&d_part, N));
+        CHK(alloc(&d_nclu, N));
         CHK(alloc(&d_stamp, N));
         CHK(alloc(&d_alive, N));
         if (P.record_node_data) CHK(alloc(&d_nd, N));
@@ -623,8 +625,10 @@ struct pxd_engine {
                                                   d_gsurv, T->clabel);
         k_mark_seen_launch(nc, T->cgraph);
         k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter, iter1);
-        k_apply<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_ckeep, T->clabel, d_cavg, d_dint, d_label,
-                                                 d_x[cur], d_y[cur], d_alive);
+        // the active node list holds the same nodes as the positions: apply in index order
+        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_nclu);
+        k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
+                                                           d_dint, d_label, d_x[cur], d_y[cur], d_alive);
         HIPC(hipGetLastError());
         // drop deactivated clusters from the position list, the cluster table and the node list
         k_pos_keepflags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pclu[0], d_ckeep, d_tA);
@@ -668,26 +672,17 @@ struct pxd_engine {
 
     void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph);
 
-    // jump until every listed member points at its root
-    int cc_flatten(const uint32_t* list, uint32_t nl) {
-        for (;;) {
-            HIPC(hipMemsetAsync(&d_cnt->jump_changed, 0, 4, st));
-            for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, list, d_root, d_cnt);
-            HIPC(hipGetLastError());
-            CHK(read_counters());
-            if (!hc.jump_changed) return 0;
-        }
-    }
-
     // Components of every CC part (clusters flagged CF_CC in table T):
-    // sampling over the first four arcs of each node, two pointer-jumping
-    // launches (trees need not end flat: roots are found by following
-    // pointers), exact linking rounds over the members outside their part's
-    // giant tree, then flattening of all members until no pointer moves.
+    // sampling rounds over the first arcs of each node, each followed by one
+    // pointer-jumping launch (trees need not end flat: roots are found by
+    // following pointers), then exact linking rounds over the members outside
+    // their part's giant tree.  The regroup pass (k_rid) leaves every tree flat.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
-        k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, 4u);
-        for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root, d_cnt);
+        for (uint32_t r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
+            k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r);
+            k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
+        }
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
         k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC);
         HIPC(hipGetLastError());
@@ -703,12 +698,14 @@ struct pxd_engine {
                 HIPC(hipGetLastError());
                 CHK(read_counters());
                 if (!hc.link_changed) break;
+                // shorten the trees the links deepened (finds stay exact without it)
+                for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_cclist2, d_root);
                 // every changed round lowers a pointer, so this bound is never reached
                 if (round >= 100000)
                     return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nu);
             }
         }
-        return cc_flatten(d_cclist, nl);
+        return 0;                     // k_rid flattens every member while it reads the roots
     }
 
     int run(int32_t* iterations) {

@@ -32,7 +32,7 @@ struct Counters {
     uint32_t nwave, nblock, nlarge;
     uint32_t ncut, ncc;
     uint32_t cclist_n;
-    uint32_t link_changed, jump_changed;
+    uint32_t link_changed;
     uint32_t pad[7];
 };
 
@@ -621,29 +621,30 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
 // k_cc_jump, to the thread's own slot): threads on different XCDs write the
 // same slots within one launch.
 //
-// Sampling: for the first `lim` same-part arcs of each node, the slot of the
-// larger of the two endpoint pointers is lowered to the smaller (atomicMin).
-// Plain reads may be stale; a stale pointer is still a smaller node of the
-// same component, so it only weakens a link.
+// Sampling, one round per arc index r (Afforest): for the r-th arc of every
+// member, if both ends are in the same part, the slot of the larger endpoint
+// pointer is lowered to the smaller (atomicMin).  A pointer-jumping launch
+// between rounds makes most endpoints of the next round share a pointer, so
+// later rounds issue few atomics.  Plain reads may be stale; a stale pointer
+// is still a smaller node of the same component, so it only weakens a link.
+constexpr uint32_t CC_SAMPLE_ROUNDS = 3;
+
 __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
                                                   const uint32_t* __restrict__ nbr,
                                                   const uint32_t* __restrict__ part,
-                                                  uint32_t* __restrict__ root, uint32_t lim) {
+                                                  uint32_t* __restrict__ root, uint32_t r) {
     const XcdSpan sp = xcd_span(nl, BS, threadIdx.x);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
-        const uint32_t start = row[u], d = min(dint[u], lim);
-        const uint32_t pp = part[u];
-        for (uint32_t j = 0; j < d; ++j) {
-            const uint32_t v = nbr[start + j];
-            if (part[v] != pp) continue;
-            const uint32_t a = root[u], b = root[v];
-            if (a == b) continue;
-            const uint32_t hi = max(a, b), lo = min(a, b);
-            if (root[hi] > lo) atomicMin(&root[hi], lo);
-        }
+        if (r >= dint[u]) continue;
+        const uint32_t v = nbr[row[u] + r];
+        if (part[v] != part[u]) continue;
+        const uint32_t a = root[u], b = root[v];
+        if (a == b) continue;
+        const uint32_t hi = max(a, b), lo = min(a, b);
+        if (root[hi] > lo) atomicMin(&root[hi], lo);
     }
 }
 
@@ -720,13 +721,10 @@ __global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __r
         atomicOr(&cnt->link_changed, 1u);
 }
 
-// pointer jumping, up to 8 steps per node and launch
+// pointer jumping, up to 8 steps per node and launch (own slot only)
 __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __restrict__ list,
-                                                uint32_t* __restrict__ root, Counters* cnt) {
-    uint32_t changed = 0;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < nl; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        if (i >= nl) continue;
+                                                uint32_t* __restrict__ root) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         const uint32_t r0 = ld_dev(&root[u]);
         uint32_t r = r0;
@@ -735,19 +733,25 @@ __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __r
             if (rr == r) break;
             r = rr;
         }
-        if (r != r0) { st_dev(&root[u], r); changed = 1; }
+        if (r != r0) st_dev(&root[u], r);
     }
-    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->jump_changed, __ATOMIC_RELAXED) == 0u)
-        atomicOr(&cnt->jump_changed, 1u);
 }
 
 // regroup key: component root for CC members, the cluster's minimum node otherwise
+// (CC members also store their root as their pointer: the trees end flat)
 __global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
                       const uint32_t* __restrict__ cflag, const uint32_t* __restrict__ cmin,
-                      const uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
+                      uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
         const uint32_t c = pclu[k];
-        key[k] = (cflag[c] & CF_CC) ? root[pnix[k]] : cmin[c];
+        if (cflag[c] & CF_CC) {
+            const uint32_t u = pnix[k];
+            const uint32_t r = cc_find(root, u);
+            st_dev(&root[u], r);      // own slot; readers see the old or the new pointer, both on u's path
+            key[k] = r;
+        } else {
+            key[k] = cmin[c];
+        }
     }
 }
 
@@ -907,14 +911,23 @@ __global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive,
     }
 }
 
-// per position: write labels, recentre survivors (x -= avg), refresh y
-__global__ void k_apply(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                        const uint32_t* __restrict__ ckeep, const int32_t* __restrict__ clabel,
-                        const double* __restrict__ cavg, const uint32_t* __restrict__ dint,
-                        int32_t* __restrict__ label, double* __restrict__ x, double* __restrict__ y,
-                        uint8_t* __restrict__ alive) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
-        const uint32_t c = pclu[k], u = pnix[k];
+// cluster of every active node: nclu[pnix[k]] = pclu[k]
+__global__ void k_node_cluster(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
+                               uint32_t* __restrict__ nclu) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
+        nclu[pnix[k]] = pclu[k];
+}
+
+// per active node, in index order: write labels, recentre survivors
+// (x -= avg), refresh y; deactivated nodes lose `alive`
+__global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ nclu,
+                              const uint32_t* __restrict__ ckeep, const int32_t* __restrict__ clabel,
+                              const double* __restrict__ cavg, const uint32_t* __restrict__ dint,
+                              int32_t* __restrict__ label, double* __restrict__ x, double* __restrict__ y,
+                              uint8_t* __restrict__ alive) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t u = alist[i];
+        const uint32_t c = nclu[u];
         label[u] = clabel[c];
         if (ckeep[c]) {
             const double v = x[u] - cavg[c];
@@ -996,15 +1009,24 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
         const uint32_t start = row[u], d = dint[u];
         const uint32_t ru = root[u];
         uint32_t kept = 0;
-        for (uint32_t j0 = 0; j0 < d; j0 += GS) {
-            const uint32_t j = j0 + lane;
-            uint32_t v = 0;
-            bool keep = false;
-            if (j < d) { v = nbr[start + j]; keep = root[v] == ru; }
-            const uint64_t bal = group_ballot<GS>(keep);
-            const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-            if (keep && kept + before != j) nbr[start + kept + before] = v;
-            kept += (uint32_t)__popcll(bal);
+        // four chunks of GS arcs loaded (indices, then roots) before the
+        // in-order compaction; a write goes to an index at most that of the
+        // arc written, all of which are already loaded
+        for (uint32_t j0 = 0; j0 < d; j0 += 4 * GS) {
+            uint32_t v[4];
+            bool keep[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) v[q] = nbr[start + min(j0 + q * GS + lane, d - 1)];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) keep[q] = (j0 + q * GS + lane < d) && root[v[q]] == ru;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint32_t j = j0 + q * GS + lane;
+                const uint64_t bal = group_ballot<GS>(keep[q]);
+                const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                if (keep[q] && kept + before != j) nbr[start + kept + before] = v[q];
+                kept += (uint32_t)__popcll(bal);
+            }
         }
         if (lane == 0 && kept != d) {
             dint[u] = kept;
