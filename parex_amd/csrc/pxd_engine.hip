@@ -590,12 +590,14 @@ struct pxd_engine {
         uint32_t nl = 0;
         if (ncc > 0) {
             k_mark<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
-                                                    d_root, d_stamp, d_tA);
+                                                    d_root, d_stamp);
+            // the CC list in node order (the active list holds the positions' nodes)
+            k_member_flags<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_stamp, iter1, d_tA);
             HIPC(hipGetLastError());
-            CHK(scan(d_tA, d_tB, na));
-            k_compact_u32<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_cclist);
+            CHK(scan(d_tA, d_tB, nalist));
+            k_compact_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tA, d_tB, d_alist[acur], d_cclist);
             HIPC(hipGetLastError());
-            CHK(read_u32(d_tB + na, &nl));
+            CHK(read_u32(d_tB + nalist, &nl));
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);

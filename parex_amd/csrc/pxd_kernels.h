@@ -591,20 +591,18 @@ __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbe
 }
 
 // cuda/devPartition.h:545-622: positions beyond the cut offset form the second
-// part.  Members of CC clusters get a part id and a fresh root; member[k]
-// flags their positions (the CC list is their scan-compacted node ids).
+// part.  Members of CC clusters get a part id, a fresh root and this
+// iteration's stamp (the CC list is built from the stamps in node order).
 __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __restrict__ pnix,
                                              const uint32_t* __restrict__ pclu,
                                              const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                                              const uint32_t* __restrict__ coff, uint32_t iter,
                                              uint32_t* __restrict__ part, uint32_t* __restrict__ root,
-                                             uint32_t* __restrict__ stamp, uint32_t* __restrict__ member) {
+                                             uint32_t* __restrict__ stamp) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
         const uint32_t c = pclu[k];
         const uint32_t f = cflag[c];
-        const bool m = (f & CF_CC) != 0u;
-        member[k] = m ? 1u : 0u;
-        if (m) {
+        if (f & CF_CC) {
             const uint32_t u = pnix[k];
             const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
             part[u] = (c << 1) | side;
@@ -612,6 +610,13 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
             stamp[u] = iter;
         }
     }
+}
+
+// f[i] = 1 for the nodes of the active list stamped in this iteration
+__global__ void k_member_flags(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ stamp,
+                               uint32_t iter, uint32_t* __restrict__ f) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        f[i] = stamp[alist[i]] == iter ? 1u : 0u;
 }
 
 // Connected components of every CC part (parent pointers `root`, members in
