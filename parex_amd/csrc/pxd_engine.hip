@@ -616,7 +616,9 @@ struct pxd_engine {
         // statistics of the (possibly new) clusters
         k_graph_first<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst);
         k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cmaxo, d_cmino, d_csum, nullptr);
-        k_stats_minmax<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cmaxo, d_cmino);
+        // positions of every cluster are in ascending walk-key order (sweep sort,
+        // then the stable regroup): min and max are the first and last keys
+        k_stats_ends<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_pnix[0], d_xkey, d_cmaxo, d_cmino);
         k_stats_scale<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK);
         k_stats_sum<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
         k_stats_final<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold,
@@ -894,6 +896,7 @@ int pxd_set_random_walk_values_f64(pxd_engine* e, const double* values) {
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(e->st));
     HIPC(hipFree(tmp));
+    e->sweep_done = false;            // the sweep order belongs to the old values
     return 0;
 }
 

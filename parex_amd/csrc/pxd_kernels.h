@@ -805,30 +805,15 @@ __global__ void k_stats_reset(uint32_t nc, uint32_t* __restrict__ cmaxo, uint32_
     }
 }
 
-// max / min of (float)x over each cluster as ordered bits (-0 < +0)
-__global__ void __launch_bounds__(BS) k_stats_minmax(uint32_t na, const uint32_t* __restrict__ pnix,
-                                                     const uint32_t* __restrict__ pclu,
-                                                     const double* __restrict__ x,
-                                                     uint32_t* __restrict__ cmaxo, uint32_t* __restrict__ cmino) {
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
-        const uint32_t k = k0 + threadIdx.x;
-        const bool ok = k < na;
-        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
-        const uint32_t o = ok ? ord_f32((float)x[pnix[k]]) : 0u;
-        const uint32_t c0 = __shfl(c, 0);
-        if (c0 == 0xFFFFFFFFu) {
-            continue;                      // whole wave past the end: nothing to add
-        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
-            uint32_t mx = ok ? o : 0u, mn = ok ? o : 0xFFFFFFFFu;
-            for (int off = WAVE / 2; off > 0; off >>= 1) {
-                mx = max(mx, __shfl_xor(mx, off));
-                mn = min(mn, __shfl_xor(mn, off));
-            }
-            if (lane_id() == 0) { atomicMax(&cmaxo[c0], mx); atomicMin(&cmino[c0], mn); }
-        } else if (ok) {
-            atomicMax(&cmaxo[c], o);
-            atomicMin(&cmino[c], o);
-        }
+// max / min of (float)x over each cluster as ordered bits (-0 < +0): the
+// positions of a cluster are sorted by that key, so they are its last and
+// first keys
+__global__ void k_stats_ends(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ pnix,
+                             const uint32_t* __restrict__ xkey, uint32_t* __restrict__ cmaxo,
+                             uint32_t* __restrict__ cmino) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        cmino[c] = xkey[pnix[cs[c]]];
+        cmaxo[c] = xkey[pnix[cs[c + 1] - 1u]];
     }
 }
 
