@@ -91,7 +91,8 @@ def cpu_baseline(graphs, budget_s):
             if shutil.which("taskset"):
                 cmd = ["taskset", "-c", "0", *cmd]
             out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=budget_s * 4 + 60).stdout
-            r = json.loads(out)
+            # the reference's CPU code prints its own warnings on stdout first
+            r = json.loads([l for l in out.splitlines() if l.startswith("{")][-1])
             return {"value": r["edges"] / r["seconds"], "unit": "edges/s", "cores": 1, "kind": "reference",
                     "sample": f"first {r['graphs']} graphs of the workload ({r['edges']} edges), "
                               f"lib/algorithms/expander_decomposition.h with Config{{0.3,1e-4,1,1}}, "

@@ -154,6 +154,53 @@ def test_end_to_end_sbm_50k():
     assert k == ok and np.array_equal(ids, oids)
 
 
+def _end_to_end_exact(g):
+    dm = _engine(g, keep_original=False, record_node_data=False)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    assert dm.expanderDecomposition() == ox.run()
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids)
+    return ids, k
+
+
+def test_torus_1024_full_size_matches_oracle():
+    """The 1024 x 1024 torus with 10,486 random long-range edges (seed 1) at full size."""
+    g = px.torus(1024, 10486, 1)
+    ids, k = _end_to_end_exact(g)
+    cut, vol, ivol, cond = px.verify_partition(g, ids, k)
+    assert vol.sum() == 2 * g.m and ivol.sum() + 2 * cut == 2 * g.m
+
+
+def test_rmat16_matches_oracle():
+    """RMAT scale 16, edge factor 16 (.57, .19, .19): hub rows and many isolated nodes."""
+    _end_to_end_exact(px.rmat(16, 16, 0.57, 0.19, 0.19, 7))
+
+
+def test_rmat22_full_size_properties():
+    """RMAT scale 22, edge factor 16, seed 7 (64M edges) at full size, where the
+    oracle takes minutes: a valid partition re-verified on the host, the same
+    result on a second run, and the same result inside a batch."""
+    g = px.rmat(22, 16, 0.57, 0.19, 0.19, 7)
+    dm = _engine(g, keep_original=True, record_node_data=False)
+    it = dm.expanderDecomposition()
+    ids, k = dm.getFinalPartition()
+    assert ids.min() == 0 and ids.max() == k - 1 and len(np.unique(ids)) == k
+    cut, vol, ivol, cond = px.verify_partition(g, ids, k)
+    assert vol.sum() == 2 * g.m and ivol.sum() + 2 * cut == 2 * g.m
+    dm.reset()
+    assert dm.expanderDecomposition() == it
+    ids2, k2 = dm.getFinalPartition()
+    assert k2 == k and np.array_equal(ids2, ids)
+    dm.close()
+    small = px.sbm(600, 5, 0.08, 0.004, 9)
+    bm = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
+    bm.initialize_batch([small, g])
+    bm.expanderDecomposition()
+    bids, bks = bm.getFinalPartition()
+    assert bm.graphIterations()[1] == it and bks[1] == k and np.array_equal(bids[small.n:], ids)
+
+
 def test_batch_mixed_density_equals_solo():
     """A batch picks its lanes per node from the batch's mean degree, a solo
     run from its own: the exact walk sum makes both give the same bits."""
