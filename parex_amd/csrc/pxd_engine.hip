@@ -126,6 +126,11 @@ struct pxd_engine {
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
     uint32_t* d_nclu = nullptr;       // cluster of every active node (apply pass)
+    // heavy rows: chunk counts, their scan, chunk -> row, chunk partial sums
+    uint32_t *d_hcnt = nullptr, *d_hoff = nullptr, *d_hrow = nullptr;
+    Fix128* d_hpart = nullptr;
+    uint32_t maxdeg = 0, heavy_thr = 0xFFFFFFFFu, heavy_cap = 0;
+    bool heavy_valid = false;
     uint32_t* d_cclist2 = nullptr;
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
@@ -157,6 +162,9 @@ struct pxd_engine {
         allocs.clear();
         lvl_sum.clear();
         lvl_scan.clear();
+        d_hcnt = d_hoff = d_hrow = nullptr;   // heavy-row buffers are allocated lazily
+        d_hpart = nullptr;
+        heavy_cap = 0;
         inited = false;
     }
 
@@ -224,6 +232,7 @@ struct pxd_engine {
         A = (uint32_t)aa;
         M_total = aa / 2;
 
+        maxdeg = 0;
         // host CSR of the whole batch (global node ids)
         std::vector<uint32_t> row(N + 1);
         std::vector<uint32_t> nbr(A);
@@ -236,8 +245,10 @@ struct pxd_engine {
                                                  (unsigned long long)j, v);
                 nbr[ab + j] = nb + v;
             }
-            for (uint32_t u = 0; u < ncnt[i]; ++u)
+            for (uint32_t u = 0; u < ncnt[i]; ++u) {
                 if (rptr[i][u + 1] < rptr[i][u]) return set_err(PXD_ERR_ARG, "graph %u: ranges decrease at %u", i, u);
+                maxdeg = std::max(maxdeg, rptr[i][u + 1] - rptr[i][u]);
+            }
         }
         row[N] = A;
 
@@ -394,6 +405,16 @@ struct pxd_engine {
         const double avg = N ? (double)A / (double)N : 0.0;
         // lanes per node: about four arcs per lane (the arc loops keep four in flight)
         gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
+        // rows longer than 32 arcs per lane leave the lane-group kernels
+        heavy_thr = maxdeg > 32u * gs ? 32u * gs : 0xFFFFFFFFu;
+        heavy_valid = false;
+        if (heavy_thr != 0xFFFFFFFFu && !d_hcnt) {
+            heavy_cap = A / HEAVY_CHUNK + A / heavy_thr + 1u;
+            CHK(alloc(&d_hcnt, N));
+            CHK(alloc(&d_hoff, (size_t)N + 1));
+            CHK(alloc(&d_hrow, heavy_cap));
+            CHK(alloc(&d_hpart, heavy_cap));
+        }
         inited = true;
         return 0;
     }
@@ -492,12 +513,12 @@ struct pxd_engine {
         const uint32_t groups_per_block = BS / GSZ;
         k_walk<GSZ><<<grid_xcd(nalist, groups_per_block), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
                                                                          d_y[cur], d_x[cur], d_x[cur ^ 1],
-                                                                         d_y[cur ^ 1], d_xkey, d_cnt);
+                                                                         d_y[cur ^ 1], d_xkey, heavy_thr, d_cnt);
     }
     template <int GSZ>
     void launch_contrib() {
         k_contrib<GSZ><<<grid_xcd(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
-                                                                     d_xkey, d_ediff);
+                                                                     d_xkey, d_ediff, heavy_thr);
     }
     template <int GSZ>
     void launch_link(const uint32_t* list, uint32_t nl) {
@@ -508,6 +529,19 @@ struct pxd_engine {
         k_disable<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
                                                                 d_x[cur], d_y[cur]);
     }
+
+    // chunk map of the active rows above heavy_thr (valid until the next cut)
+    int heavy_map() {
+        if (heavy_thr == 0xFFFFFFFFu || heavy_valid || nalist == 0) return 0;
+        k_heavy_count<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, heavy_thr, d_hcnt);
+        HIPC(hipGetLastError());
+        CHK(scan(d_hcnt, d_hoff, nalist));
+        k_heavy_expand<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_hcnt, d_hoff, d_hrow);
+        HIPC(hipGetLastError());
+        heavy_valid = true;
+        return 0;
+    }
+    uint32_t heavy_grid() const { return std::min(heavy_cap, 4096u); }
 
 #define PXD_DISPATCH_GS(fn, ...)                          \
     switch (gs) {                                         \
@@ -522,8 +556,18 @@ struct pxd_engine {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (nalist) {
             if (walk_timing) HIPC(hipEventRecord(ev0, st));
+            CHK(heavy_map());
             PXD_DISPATCH_GS(launch_walk);
             HIPC(hipGetLastError());
+            if (heavy_thr != 0xFFFFFFFFu) {
+                const uint32_t* H = d_hoff + nalist;   // total chunk count, on the device
+                k_walk_heavy<<<heavy_grid(), BS, 0, st>>>(H, d_hrow, d_hoff, d_alist[acur], d_row, d_dint, d_nbr,
+                                                           d_y[cur], d_hpart, d_cnt);
+                k_walk_heavy_finish<<<grid_for(heavy_cap, BS), BS, 0, st>>>(H, d_hrow, d_hoff, d_alist[acur], d_dint,
+                                                                             d_hpart, d_x[cur], d_x[cur ^ 1],
+                                                                             d_y[cur ^ 1], d_xkey, d_cnt);
+                HIPC(hipGetLastError());
+            }
             if (walk_timing) {
                 HIPC(hipEventRecord(ev1, st));
                 HIPC(hipEventSynchronize(ev1));
@@ -559,8 +603,14 @@ struct pxd_engine {
         k_gather<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
         HIPC(hipGetLastError());
         CHK(seg_sort(T.cs, nc));
+        CHK(heavy_map());
         PXD_DISPATCH_GS(launch_contrib);
         HIPC(hipGetLastError());
+        if (heavy_thr != 0xFFFFFFFFu) {
+            k_contrib_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nalist, d_hrow, d_hoff, d_alist[acur], d_row, d_dint,
+                                                          d_nbr, d_xkey, d_ediff);
+            HIPC(hipGetLastError());
+        }
         k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, nullptr, nullptr, nullptr, d_cbest);
         const uint32_t nb = cdiv(na, BS * SCAN_ITEMS);
         k_sweep_a<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, d_ediff, d_dint, d_tail_ed, d_tail_vol, d_hasst);
@@ -584,6 +634,7 @@ struct pxd_engine {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (!sweep_done) return set_err(PXD_ERR_ARG, "cutClusters needs a preceding computeSweepCuts");
         sweep_done = false;
+        heavy_valid = false;          // the cut changes the active list and internal degrees
         ++iter1;
         if (na == 0 || nc == 0) return 0;
         ClusterTable* T = &ct[ctc];

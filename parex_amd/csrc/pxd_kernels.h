@@ -96,6 +96,19 @@ __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict_
 // index order: x'[u] = 0.9 * sum_{v in valid(u)} x[v]/dInt[v] + 0.1 * x[u]
 // (cuda/devRandomWalk.h:117-203).  The neighbour sum is exact (fixed point),
 // so its bits do not depend on GS or on the lane that adds a term.
+// x' = s * (1 - 0.1) + x * 0.1 with s the exact neighbour sum, y' = x'/dInt,
+// walk key (cuda/devRandomWalk.h:186-187)
+__device__ __forceinline__ void walk_finish(uint32_t u, uint32_t d, Fix128 acc, const double* __restrict__ xc,
+                                            double* __restrict__ xn, double* __restrict__ yn,
+                                            uint32_t* __restrict__ xkey, uint32_t& ovf) {
+    if (d > (1u << 24)) ovf = 1;
+    const double s = fix_to_f64(acc);
+    const double v = s * (1.0 - RW_STAY) + xc[u] * RW_STAY;
+    xn[u] = v;
+    yn[u] = d ? v / (double)d : 0.0;
+    xkey[u] = ord_f32((float)v);
+}
+
 template <int GS>
 __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __restrict__ alist,
                                              const uint32_t* __restrict__ row,
@@ -104,13 +117,14 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
                                              const double* __restrict__ y,
                                              const double* __restrict__ xc, double* __restrict__ xn,
                                              double* __restrict__ yn, uint32_t* __restrict__ xkey,
-                                             Counters* cnt) {
+                                             uint32_t heavy, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
     uint32_t ovf = 0;
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
+        if (d > heavy) continue;      // rows above the threshold go to k_walk_heavy
         Fix128 acc = {0ull, 0ull};
         // four independent index loads, then four value gathers in flight per
         // lane; slots past the row end re-read the row's last arc (no branch,
@@ -129,14 +143,7 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
             fix_add_term(acc, b3 ? y3 : 0.0, ovf);
         }
         acc = group_sum_fix<GS>(acc);
-        if (lane == 0) {
-            if (d > (1u << 24)) ovf = 1;
-            const double s = fix_to_f64(acc);
-            const double v = s * (1.0 - RW_STAY) + xc[u] * RW_STAY;
-            xn[u] = v;
-            yn[u] = d ? v / (double)d : 0.0;
-            xkey[u] = ord_f32((float)v);
-        }
+        if (lane == 0) walk_finish(u, d, acc, xc, xn, yn, xkey, ovf);
     }
     if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
 }
@@ -152,12 +159,16 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ xkey,
-                                                int32_t* __restrict__ ediff) {
+                                                int32_t* __restrict__ ediff, uint32_t heavy) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
+        if (d > heavy) {              // k_contrib_heavy adds the chunks' counts
+            if (lane == 0) ediff[u] = 0;
+            continue;
+        }
         const uint32_t mk = xkey[u];
         int32_t c = 0;
         const uint32_t* p = nbr + start;
@@ -179,6 +190,149 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
         }
         c = group_sum<GS>(c);
         if (lane == 0) ediff[u] = c;
+    }
+}
+
+// ------------------------------------------------------------------ heavy rows
+
+// Rows above the light kernels' threshold are split into chunks of
+// HEAVY_CHUNK arcs, one workgroup per chunk.  hcnt[i] = chunk count of the
+// i-th active node (0 for light rows); its exclusive scan hoff gives each
+// heavy row its chunk range, hrow[chunk] = the row's index in the list.
+constexpr uint32_t HEAVY_CHUNK = BS * 4;
+
+__global__ void k_heavy_count(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ dint,
+                              uint32_t heavy, uint32_t* __restrict__ hcnt) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t d = dint[alist[i]];
+        hcnt[i] = d > heavy ? (d + HEAVY_CHUNK - 1u) / HEAVY_CHUNK : 0u;
+    }
+}
+
+__global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, const uint32_t* __restrict__ hoff,
+                               uint32_t* __restrict__ hrow) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = hcnt[i], o = hoff[i];
+        for (uint32_t k = 0; k < c; ++k) hrow[o + k] = i;
+    }
+}
+
+// sum of a Fix128 over the workgroup, valid in thread 0
+__device__ __forceinline__ Fix128 block_sum_fix(Fix128 a, Fix128* lds) {
+    a = group_sum_fix<WAVE>(a);
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t w = 1; w < BS / WAVE; ++w) fix_add128(a, lds[w].lo, lds[w].hi);
+    return a;
+}
+
+// partial exact sum of one chunk of a heavy row; chunk count read on the device
+__global__ void __launch_bounds__(BS) k_walk_heavy(const uint32_t* __restrict__ nchunks,
+                                                   const uint32_t* __restrict__ hrow,
+                                                   const uint32_t* __restrict__ hoff,
+                                                   const uint32_t* __restrict__ alist,
+                                                   const uint32_t* __restrict__ row,
+                                                   const uint32_t* __restrict__ dint,
+                                                   const uint32_t* __restrict__ nbr,
+                                                   const double* __restrict__ y, Fix128* __restrict__ part,
+                                                   Counters* cnt) {
+    __shared__ Fix128 lds[BS / WAVE];
+    const uint32_t H = *nchunks;
+    uint32_t ovf = 0;
+    for (uint32_t b = blockIdx.x; b < H; b += gridDim.x) {
+        const uint32_t i = hrow[b];
+        const uint32_t u = alist[i];
+        const uint32_t d = dint[u];
+        const uint32_t j0 = (b - hoff[i]) * HEAVY_CHUNK;
+        const uint32_t* p = nbr + row[u];
+        uint32_t v[4];
+        bool ok[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t j = j0 + q * BS + threadIdx.x;
+            ok[q] = j < d;
+            v[q] = p[min(j, d - 1u)];
+        }
+        double yv[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) yv[q] = y[v[q]];
+        Fix128 acc = {0ull, 0ull};
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) fix_add_term(acc, ok[q] ? yv[q] : 0.0, ovf);
+        acc = block_sum_fix(acc, lds);
+        if (threadIdx.x == 0) part[b] = acc;
+        __syncthreads();              // lds reused by the next chunk
+    }
+    if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
+}
+
+// per heavy row (its first chunk): exact sum of the chunk partials, then the
+// same rounding and stores as the light kernel
+__global__ void k_walk_heavy_finish(const uint32_t* __restrict__ nchunks, const uint32_t* __restrict__ hrow,
+                                    const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ alist,
+                                    const uint32_t* __restrict__ dint, const Fix128* __restrict__ part,
+                                    const double* __restrict__ xc, double* __restrict__ xn,
+                                    double* __restrict__ yn, uint32_t* __restrict__ xkey, Counters* cnt) {
+    const uint32_t H = *nchunks;
+    uint32_t ovf = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < H; b += gridDim.x * blockDim.x) {
+        const uint32_t i = hrow[b];
+        if (hoff[i] != b) continue;
+        const uint32_t u = alist[i];
+        const uint32_t d = dint[u];
+        const uint32_t nch = (d + HEAVY_CHUNK - 1u) / HEAVY_CHUNK;
+        Fix128 acc = {0ull, 0ull};
+        for (uint32_t k = 0; k < nch; ++k) fix_add128(acc, part[b + k].lo, part[b + k].hi);
+        walk_finish(u, d, acc, xc, xn, yn, xkey, ovf);
+    }
+    if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
+}
+
+// cut contribution of one chunk of a heavy row, added to ediff[u] (exact integers)
+__global__ void __launch_bounds__(BS) k_contrib_heavy(const uint32_t* __restrict__ nchunks,
+                                                      const uint32_t* __restrict__ hrow,
+                                                      const uint32_t* __restrict__ hoff,
+                                                      const uint32_t* __restrict__ alist,
+                                                      const uint32_t* __restrict__ row,
+                                                      const uint32_t* __restrict__ dint,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ xkey,
+                                                      int32_t* __restrict__ ediff) {
+    __shared__ int32_t lds[BS / WAVE];
+    const uint32_t H = *nchunks;
+    for (uint32_t b = blockIdx.x; b < H; b += gridDim.x) {
+        const uint32_t i = hrow[b];
+        const uint32_t u = alist[i];
+        const uint32_t d = dint[u];
+        const uint32_t mk = xkey[u];
+        const uint32_t j0 = (b - hoff[i]) * HEAVY_CHUNK;
+        const uint32_t* p = nbr + row[u];
+        uint32_t v[4], k[4];
+        bool ok[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t j = j0 + q * BS + threadIdx.x;
+            ok[q] = j < d;
+            v[q] = p[min(j, d - 1u)];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) k[q] = xkey[v[q]];
+        int32_t c = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const bool before = (k[q] != mk) ? (k[q] < mk) : (v[q] < u);
+            if (ok[q]) c += before ? -1 : 1;
+        }
+        c = group_sum<WAVE>(c);
+        if (lane_id() == 0) lds[threadIdx.x / WAVE] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t t = 0;
+            for (uint32_t w = 0; w < BS / WAVE; ++w) t += lds[w];
+            atomicAdd(&ediff[u], t);
+        }
+        __syncthreads();
     }
 }
 
