@@ -31,7 +31,9 @@ typedef enum {
     PXD_ERR_HIP = -2,        /* HIP runtime error */
     PXD_ERR_OOM = -3,        /* device allocation failed */
     PXD_ERR_NUMERIC = -4,    /* walk value outside the exact fixed-point range */
-    PXD_ERR_STATE = -5       /* internal invariant violated */
+    PXD_ERR_STATE = -5,      /* internal invariant violated */
+    PXD_FULL = 1,            /* serving: the graph does not fit the free capacity now */
+    PXD_EMPTY = 1            /* serving: no finished result is waiting */
 } pxd_status;
 
 /* lib/core/definitions.h:57-84 -- identical byte layouts */
@@ -143,6 +145,34 @@ int pxd_last_run_ms(pxd_engine* e, double* ms);
  * bytes (4+8 per valid arc + 32 per active node) summed over those launches. */
 int pxd_enable_walk_timing(pxd_engine* e, int enable);
 int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes);
+
+/* ------------------------------------------------------------------ serving
+ * No counterpart in the reference (cuda/interface.h runs one graph per
+ * manager, main.cpp:44-50 one file per process).  Graphs join a running engine
+ * between iterations and leave when they finish: continuous batching.  Each
+ * graph's result equals its solo pxd_expander_decomposition result.
+ * Capacities fix the device footprint: node ranges, arc ranges and graph
+ * slots are taken first fit and reused. */
+
+/* Engine with room for node_capacity nodes (< 2^31), arc_capacity arcs
+ * (2 per undirected edge, < 2^32) and graph_slots graphs in flight. */
+int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t arc_capacity, uint32_t graph_slots,
+                     pxd_engine** out);
+/* Admit one graph (CSR as for pxd_initialize) under the caller's tag.
+ * PXD_FULL: it does not fit now (retry after steps free room); the host
+ * buffers may be reused once the call returns. */
+int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
+                    uint64_t tag, int32_t* slot);
+/* One iteration (walk, sweep, cut) of every graph in flight; *finished =
+ * graphs that completed in it (their results wait for pxd_serve_take). */
+int pxd_serve_step(pxd_engine* e, uint32_t* finished);
+/* Tag and node count of the oldest waiting result; PXD_EMPTY if none. */
+int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n);
+/* Pop that result: cluster id per node (ids_out[n]), cluster count and the
+ * iteration count its solo run reports (cuda/device.h:343-350). */
+int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations);
+int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
+                    uint32_t* active_nodes, uint32_t* waiting_results);
 
 #ifdef __cplusplus
 }

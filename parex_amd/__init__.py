@@ -130,6 +130,15 @@ def pxd() -> C.CDLL:
         lib.pxd_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
         lib.pxd_enable_walk_timing.argtypes = [P, C.c_int]
         lib.pxd_walk_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.pxd_serve_create.argtypes = [C.POINTER(_Params), C.c_uint32, C.c_uint64, C.c_uint32,
+                                         C.POINTER(C.c_void_p)]
+        lib.pxd_serve_admit.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint64,
+                                        C.POINTER(C.c_int32)]
+        lib.pxd_serve_step.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.pxd_serve_next.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        lib.pxd_serve_take.argtypes = [P, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        lib.pxd_serve_stats.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib._typed = True
     return lib
 
@@ -143,7 +152,11 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
+    "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_take", "pxd_serve_stats",
 ]
+
+PXD_FULL = 1       # pxd_serve_admit: no room right now
+PXD_EMPTY = 1      # pxd_serve_next / pxd_serve_take: no finished result waiting
 
 
 # --------------------------------------------------------------------------- graphs
@@ -259,6 +272,68 @@ class Params:
     keep_original: bool = True
     record_node_data: bool = True
     device: int = 0
+
+
+class Server:
+    """Continuous-batching decomposition service over one engine: graphs are
+    admitted between iterations while others are in flight (pxd_serve_*),
+    each result equal to the graph's solo decomposition."""
+
+    def __init__(self, node_capacity: int, arc_capacity: int, graph_slots: int, params: Params | None = None):
+        self._lib = pxd()
+        self.params = params or Params()
+        p = _Params(self.params.seed, self.params.sc_threshold, self.params.rw_threshold, self.params.max_iter,
+                    0, int(self.params.record_node_data), self.params.device)
+        h = C.c_void_p()
+        self._check(self._lib.pxd_serve_create(C.byref(p), node_capacity, arc_capacity, graph_slots, C.byref(h)))
+        self._h = h
+
+    def _check(self, rc: int):
+        if rc < 0:
+            raise PxdError(rc, self._lib.pxd_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pxd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, graph: Graph, tag: int) -> bool:
+        """True if admitted, False if it does not fit the free capacity now."""
+        ranges = np.ascontiguousarray(graph.ranges, np.uint32)
+        edges = np.ascontiguousarray(graph.edges, np.uint32)
+        slot = C.c_int32()
+        rc = self._check(self._lib.pxd_serve_admit(self._h, graph.n, graph.m, ranges, edges, tag, C.byref(slot)))
+        return rc == 0
+
+    def step(self) -> int:
+        """One iteration of every graph in flight; returns how many finished."""
+        f = C.c_uint32()
+        self._check(self._lib.pxd_serve_step(self._h, C.byref(f)))
+        return f.value
+
+    def results(self):
+        """Pop every waiting result as (tag, ids, num_clusters, iterations)."""
+        out = []
+        tag, n = C.c_uint64(), C.c_uint32()
+        while self._check(self._lib.pxd_serve_next(self._h, C.byref(tag), C.byref(n))) == 0:
+            ids = np.empty(n.value, np.int32)
+            k, it = C.c_int32(), C.c_int32()
+            self._check(self._lib.pxd_serve_take(self._h, ids, C.byref(k), C.byref(it)))
+            out.append((tag.value, ids, k.value, it.value))
+        return out
+
+    def stats(self):
+        g, nn, aa, na, w = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.pxd_serve_stats(self._h, C.byref(g), C.byref(nn), C.byref(aa), C.byref(na), C.byref(w)))
+        return {"live_graphs": g.value, "live_nodes": nn.value, "live_arcs": aa.value, "active_nodes": na.value,
+                "waiting_results": w.value}
 
 
 class DeviceManager:

@@ -11,6 +11,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -69,6 +71,30 @@ struct ClusterTable {
     ClusterCols cols() const { return ClusterCols{cs, cgraph, cmin, cflag, cvol, clabel}; }
 };
 
+// cuda/devPartition.h:798-823 for one graph's labels: id = rank of the label
+// among the distinct labels (sort, unique, rank); counting over the label
+// range when it is small, a sort otherwise.  Returns the number of clusters.
+static int32_t dense_rank(const int32_t* lab, uint32_t n, int32_t* ids) {
+    if (n == 0) return 0;
+    int32_t lo = lab[0], hi = lab[0];
+    for (uint32_t i = 1; i < n; ++i) { lo = std::min(lo, lab[i]); hi = std::max(hi, lab[i]); }
+    const uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo) + 1u;
+    if (range <= 4ull * n + 64u) {
+        std::vector<int32_t> rank(range, -1);
+        for (uint32_t i = 0; i < n; ++i) rank[lab[i] - lo] = 0;
+        int32_t k = 0;
+        for (uint64_t r = 0; r < range; ++r)
+            if (rank[r] == 0) rank[r] = ++k;
+        for (uint32_t i = 0; i < n; ++i) ids[i] = rank[lab[i] - lo] - 1;
+        return k;
+    }
+    std::vector<int32_t> u(lab, lab + n);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    for (uint32_t i = 0; i < n; ++i) ids[i] = (int32_t)(std::lower_bound(u.begin(), u.end(), lab[i]) - u.begin());
+    return (int32_t)u.size();
+}
+
 struct pxd_engine {
     pxd_params P{};
     int dev = 0;
@@ -126,6 +152,8 @@ struct pxd_engine {
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
     uint32_t* d_nclu = nullptr;       // cluster of every active node (apply pass)
+    uint32_t* d_ngraph = nullptr;     // graph (slot) of every node
+    uint32_t* d_gstart = nullptr;     // iteration a graph joined at (0 for a batch)
     // heavy rows: chunk counts, their scan, chunk -> row, chunk partial sums
     uint32_t *d_hcnt = nullptr, *d_hoff = nullptr, *d_hrow = nullptr;
     Fix128* d_hpart = nullptr;
@@ -204,10 +232,105 @@ struct pxd_engine {
         return 0;
     }
 
+    // device arrays for N nodes, A arcs and G graphs (capacities when serving)
+    int allocate() {
+        const size_t n1 = (size_t)N + 1;
+        CHK(alloc(&d_row, n1));
+        CHK(alloc(&d_deg, N));
+        CHK(alloc(&d_nbr, A));
+        if (P.keep_original) CHK(alloc(&d_nbr0, A));
+        CHK(alloc(&d_gnode, G + 1));
+        CHK(alloc(&d_dint, N));
+        CHK(alloc(&d_label, N));
+        for (int b = 0; b < 2; ++b) {
+            CHK(alloc(&d_x[b], N));
+            CHK(alloc(&d_y[b], N));
+            CHK(alloc(&d_pnix[b], N));
+            CHK(alloc(&d_pkey[b], N));
+            CHK(alloc(&d_pclu[b], N));
+            CHK(alloc(&d_alist[b], N));
+            CHK(alloc(&ct[b].cs, n1 + 1));
+            CHK(alloc(&ct[b].cgraph, n1));
+            CHK(alloc(&ct[b].cmin, n1));
+            CHK(alloc(&ct[b].cflag, n1));
+            CHK(alloc(&ct[b].cvol, n1));
+            CHK(alloc(&ct[b].clabel, n1));
+        }
+        CHK(alloc(&d_xkey, N));
+        CHK(alloc(&d_ediff, N));
+        CHK(alloc(&d_root, N));
+        CHK(alloc(&d_part, N));
+        CHK(alloc(&d_nclu, N));
+        CHK(alloc(&d_ngraph, N));
+        CHK(alloc(&d_stamp, N));
+        CHK(alloc(&d_alive, N));
+        if (P.record_node_data) CHK(alloc(&d_nd, N));
+        CHK(alloc(&d_coff, n1));
+        CHK(alloc(&d_ckeep, n1));
+        CHK(alloc(&d_kscan, n1 + 1));
+        CHK(alloc(&d_cmaxo, n1));
+        CHK(alloc(&d_cmino, n1));
+        CHK(alloc(&d_cK, n1));
+        CHK(alloc(&d_csp, n1));
+        CHK(alloc(&d_cavg, n1));
+        CHK(alloc(&d_cbest, n1));
+        CHK(alloc(&d_csum, n1));
+        CHK(alloc(&d_gfirst, G + 1));
+        CHK(alloc(&d_gsmall, G));
+        CHK(alloc(&d_gsmall_next, G));
+        CHK(alloc(&d_giter, G));
+        CHK(alloc(&d_gsurv, G));
+        CHK(alloc(&d_gseen, G));
+        CHK(alloc(&d_gstart, G));
+        CHK(alloc(&d_tA, n1 + 1));
+        CHK(alloc(&d_tB, n1 + 1));
+        CHK(alloc(&d_tC, n1 + 1));
+        CHK(alloc(&d_cclist, N));
+        CHK(alloc(&d_lwave, n1));
+        CHK(alloc(&d_lblock, n1));
+        CHK(alloc(&d_llarge, n1));
+        ls_cap = N / TILE + 2;
+        tiles_cap = 2 * (N / TILE) + 4;
+        CHK(alloc(&d_lstart, ls_cap));
+        CHK(alloc(&d_llen, ls_cap));
+        CHK(alloc(&d_ls, ls_cap));
+        CHK(alloc(&d_tseg, tiles_cap));
+        CHK(alloc(&d_tidx, tiles_cap));
+        CHK(alloc(&d_hist, (size_t)tiles_cap * 256 + 1));
+        CHK(alloc(&d_hscan, (size_t)tiles_cap * 256 + 2));
+        const uint32_t nbk = cdiv(n1, BS * SCAN_ITEMS) + 1;
+        CHK(alloc(&d_tail_ed, nbk));
+        CHK(alloc(&d_tail_vol, nbk));
+        CHK(alloc(&d_hasst, nbk));
+        CHK(alloc(&d_carry_ed, nbk));
+        CHK(alloc(&d_carry_vol, nbk));
+        {
+            uint64_t len = std::max<uint64_t>(n1 + 1, (uint64_t)tiles_cap * 256 + 2);
+            while (true) {
+                uint32_t nb = cdiv(len, BS * SCAN_ITEMS);
+                uint32_t *s = nullptr, *c = nullptr;
+                CHK(alloc(&s, nb + 1));
+                CHK(alloc(&c, nb + 2));
+                lvl_sum.push_back(s);
+                lvl_scan.push_back(c);
+                if (nb <= 1) break;
+                len = nb;
+            }
+        }
+        CHK(alloc(&d_oneseg, 2));
+        CHK(alloc(&d_gclu, G));
+        CHK(alloc(&d_giant, 2 * n1 + 2));
+        CHK(alloc(&d_cclist2, N));
+        CHK(alloc(&d_u64, 1));
+        CHK(alloc(&d_cnt, 1));
+        return 0;
+    }
+
     // ---------------------------------------------------------------- setup
     int setup(uint32_t g, const uint32_t* ncnt, const uint32_t* ecnt, const uint32_t* const* rptr,
               const uint32_t* const* eptr) {
         release();
+        serving = false;
         HIPC(hipSetDevice(dev));
         G = g;
         h_gnode.assign(G + 1, 0);
@@ -252,93 +375,8 @@ struct pxd_engine {
         }
         row[N] = A;
 
+        CHK(allocate());
         const size_t n1 = (size_t)N + 1;
-        CHK(alloc(&d_row, n1));
-        CHK(alloc(&d_deg, N));
-        CHK(alloc(&d_nbr, A));
-        if (P.keep_original) CHK(alloc(&d_nbr0, A));
-        CHK(alloc(&d_gnode, G + 1));
-        CHK(alloc(&d_dint, N));
-        CHK(alloc(&d_label, N));
-        for (int b = 0; b < 2; ++b) {
-            CHK(alloc(&d_x[b], N));
-            CHK(alloc(&d_y[b], N));
-            CHK(alloc(&d_pnix[b], N));
-            CHK(alloc(&d_pkey[b], N));
-            CHK(alloc(&d_pclu[b], N));
-            CHK(alloc(&d_alist[b], N));
-            CHK(alloc(&ct[b].cs, n1 + 1));
-            CHK(alloc(&ct[b].cgraph, n1));
-            CHK(alloc(&ct[b].cmin, n1));
-            CHK(alloc(&ct[b].cflag, n1));
-            CHK(alloc(&ct[b].cvol, n1));
-            CHK(alloc(&ct[b].clabel, n1));
-        }
-        CHK(alloc(&d_xkey, N));
-        CHK(alloc(&d_ediff, N));
-        CHK(alloc(&d_root, N));
-        CHK(alloc(&d_part, N));
-        CHK(alloc(&d_nclu, N));
-        CHK(alloc(&d_stamp, N));
-        CHK(alloc(&d_alive, N));
-        if (P.record_node_data) CHK(alloc(&d_nd, N));
-        CHK(alloc(&d_coff, n1));
-        CHK(alloc(&d_ckeep, n1));
-        CHK(alloc(&d_kscan, n1 + 1));
-        CHK(alloc(&d_cmaxo, n1));
-        CHK(alloc(&d_cmino, n1));
-        CHK(alloc(&d_cK, n1));
-        CHK(alloc(&d_csp, n1));
-        CHK(alloc(&d_cavg, n1));
-        CHK(alloc(&d_cbest, n1));
-        CHK(alloc(&d_csum, n1));
-        CHK(alloc(&d_gfirst, G + 1));
-        CHK(alloc(&d_gsmall, G));
-        CHK(alloc(&d_gsmall_next, G));
-        CHK(alloc(&d_giter, G));
-        CHK(alloc(&d_gsurv, G));
-        CHK(alloc(&d_gseen, G));
-        CHK(alloc(&d_tA, n1 + 1));
-        CHK(alloc(&d_tB, n1 + 1));
-        CHK(alloc(&d_tC, n1 + 1));
-        CHK(alloc(&d_cclist, N));
-        CHK(alloc(&d_lwave, n1));
-        CHK(alloc(&d_lblock, n1));
-        CHK(alloc(&d_llarge, n1));
-        ls_cap = N / TILE + 2;
-        tiles_cap = 2 * (N / TILE) + 4;
-        CHK(alloc(&d_lstart, ls_cap));
-        CHK(alloc(&d_llen, ls_cap));
-        CHK(alloc(&d_ls, ls_cap));
-        CHK(alloc(&d_tseg, tiles_cap));
-        CHK(alloc(&d_tidx, tiles_cap));
-        CHK(alloc(&d_hist, (size_t)tiles_cap * 256 + 1));
-        CHK(alloc(&d_hscan, (size_t)tiles_cap * 256 + 2));
-        const uint32_t nbk = cdiv(n1, BS * SCAN_ITEMS) + 1;
-        CHK(alloc(&d_tail_ed, nbk));
-        CHK(alloc(&d_tail_vol, nbk));
-        CHK(alloc(&d_hasst, nbk));
-        CHK(alloc(&d_carry_ed, nbk));
-        CHK(alloc(&d_carry_vol, nbk));
-        {
-            uint64_t len = std::max<uint64_t>(n1 + 1, (uint64_t)tiles_cap * 256 + 2);
-            while (true) {
-                uint32_t nb = cdiv(len, BS * SCAN_ITEMS);
-                uint32_t *s = nullptr, *c = nullptr;
-                CHK(alloc(&s, nb + 1));
-                CHK(alloc(&c, nb + 2));
-                lvl_sum.push_back(s);
-                lvl_scan.push_back(c);
-                if (nb <= 1) break;
-                len = nb;
-            }
-        }
-        CHK(alloc(&d_oneseg, 2));
-        CHK(alloc(&d_gclu, G));
-        CHK(alloc(&d_giant, 2 * n1 + 2));
-        CHK(alloc(&d_cclist2, N));
-        CHK(alloc(&d_u64, 1));
-        CHK(alloc(&d_cnt, 1));
 
         HIPC(hipMemcpyAsync(d_row, row.data(), n1 * 4, hipMemcpyHostToDevice, st));
         if (A) HIPC(hipMemcpyAsync(d_nbr, nbr.data(), (size_t)A * 4, hipMemcpyHostToDevice, st));
@@ -353,6 +391,7 @@ struct pxd_engine {
     // back to the state right after initialize, from the device-resident input CSR
     int reset() {
         if (!inited && h_gnode.empty()) return set_err(PXD_ERR_ARG, "engine not initialized");
+        if (serving) return set_err(PXD_ERR_ARG, "a serving engine has no reset");
         if (!P.keep_original) return set_err(PXD_ERR_ARG, "reset needs keep_original = 1");
         HIPC(hipSetDevice(dev));
         if (A) HIPC(hipMemcpyAsync(d_nbr, d_nbr0, (size_t)A * 4, hipMemcpyDeviceToDevice, st));
@@ -388,11 +427,12 @@ struct pxd_engine {
         HIPC(hipMemsetAsync(d_giter, 0, G * 4, st));
         HIPC(hipMemsetAsync(d_gsurv, 0, G * 4, st));
         HIPC(hipMemsetAsync(d_gseen, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gstart, 0, G * 4, st));
         cur = acur = pcur = 0;
         if (N) {
             k_init_nodes<<<grid_for(N, BS), BS, 0, st>>>(N, G, d_gnode, d_row, P.seed, d_deg, d_dint, d_x[0], d_y[0],
                                                           d_xkey, d_label, d_alive, d_stamp, d_pnix[0], d_alist[0],
-                                                          d_pclu[0], d_gclu);
+                                                          d_pclu[0], d_gclu, d_ngraph);
             HIPC(hipGetLastError());
         }
         CHK(sync());
@@ -659,7 +699,7 @@ struct pxd_engine {
             k_run_flags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pkey[0], d_tA);
             HIPC(hipGetLastError());
             CHK(scan(d_tA, d_tB, na));
-            k_build_clusters<<<grid_for(na, BS), BS, 0, st>>>(na, G, d_gnode, d_pkey[0], d_tA, d_tB, T->cs, T->cmin,
+            k_build_clusters<<<grid_for(na, BS), BS, 0, st>>>(na, d_ngraph, d_pkey[0], d_tA, d_tB, T->cs, T->cmin,
                                                               T->cgraph, T->cflag, d_pclu[0]);
             HIPC(hipGetLastError());
             CHK(read_u32(d_tB + na, &nc));
@@ -679,7 +719,8 @@ struct pxd_engine {
         k_labels<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst, d_ckeep, d_kscan, d_gsmall, d_gsmall_next,
                                                   d_gsurv, T->clabel);
         k_mark_seen_launch(nc, T->cgraph);
-        k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter, iter1);
+        k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter,
+                                                       d_gstart, iter1);
         // the active node list holds the same nodes as the positions: apply in index order
         k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_nclu);
         k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
@@ -763,8 +804,211 @@ struct pxd_engine {
         return 0;                     // k_rid flattens every member while it reads the roots
     }
 
+    // ---------------------------------------------------------------- serving
+    // Graphs join a running engine between iterations and leave when they
+    // finish.  Node ranges, arc ranges and graph slots are taken first fit
+    // from fixed capacities (admission by device footprint) and reused.
+    bool serving = false;
+    std::map<uint64_t, uint64_t> free_nodes, free_arcs;   // start -> length
+    struct Slot {
+        uint64_t tag = 0;
+        uint32_t base = 0, n = 0, abase = 0, arcs = 0;
+        bool used = false;
+    };
+    std::vector<Slot> slots;
+    std::vector<uint32_t> free_slots;
+    uint32_t live_graphs = 0;
+    uint64_t live_nodes = 0, live_arcs = 0;
+    struct Done {
+        uint64_t tag;
+        int32_t k, it;
+        std::vector<int32_t> ids;
+    };
+    std::deque<Done> done;
+    uint32_t* d_stage = nullptr;      // ranges of the graph being admitted
+    std::vector<int32_t> h_giter;
+
+    static bool take_range(std::map<uint64_t, uint64_t>& fl, uint64_t len, uint64_t* start) {
+        if (len == 0) { *start = 0; return true; }
+        for (auto it = fl.begin(); it != fl.end(); ++it) {
+            if (it->second < len) continue;
+            *start = it->first;
+            const uint64_t rest = it->second - len, ns = it->first + len;
+            fl.erase(it);
+            if (rest) fl[ns] = rest;
+            return true;
+        }
+        return false;
+    }
+    static void give_range(std::map<uint64_t, uint64_t>& fl, uint64_t start, uint64_t len) {
+        if (len == 0) return;
+        auto it = fl.emplace(start, len).first;
+        auto nx = std::next(it);
+        if (nx != fl.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            fl.erase(nx);
+        }
+        if (it != fl.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                fl.erase(it);
+            }
+        }
+    }
+
+    int serve_create(uint32_t node_cap, uint64_t arc_cap, uint32_t graph_cap) {
+        release();
+        HIPC(hipSetDevice(dev));
+        if (node_cap == 0 || node_cap >= 0x7FFFFFFFu || arc_cap >= 0xFFFFFFFFull || graph_cap == 0)
+            return set_err(PXD_ERR_ARG, "serving capacities out of range (nodes < 2^31, arcs < 2^32, slots > 0)");
+        N = node_cap;
+        A = (uint32_t)arc_cap;
+        G = graph_cap;
+        M_total = 0;
+        h_gnode.assign(G + 1, 0);
+        h_garc.assign(G + 1, 0);
+        h_gm.assign(G, 0);
+        P.keep_original = 0;          // no reset: the input CSR is not kept
+        CHK(allocate());
+        CHK(alloc(&d_stage, (size_t)N + 1));
+        // heavy-row buffers sized for the smallest threshold (4 lanes per node)
+        heavy_cap = A / HEAVY_CHUNK + A / 128u + 1u;
+        CHK(alloc(&d_hcnt, N));
+        CHK(alloc(&d_hoff, (size_t)N + 1));
+        CHK(alloc(&d_hrow, heavy_cap));
+        CHK(alloc(&d_hpart, heavy_cap));
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
+        HIPC(hipMemsetAsync(d_gsmall, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gsmall_next, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_giter, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gsurv, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gseen, 0, G * 4, st));
+        HIPC(hipMemsetAsync(d_gstart, 0, G * 4, st));
+        free_nodes.clear();
+        free_nodes[0] = N;
+        free_arcs.clear();
+        if (A) free_arcs[0] = A;
+        slots.assign(G, Slot{});
+        free_slots.clear();
+        for (uint32_t g = G; g-- > 0;) free_slots.push_back(g);
+        live_graphs = 0;
+        live_nodes = live_arcs = 0;
+        done.clear();
+        h_giter.assign(G, 0);
+        cur = acur = pcur = 0;
+        ctc = 0;
+        na = nalist = nc = 0;
+        ncut = ncc = 0;
+        iter1 = 0;
+        sweep_done = false;
+        maxdeg = 0;
+        gs = 4;
+        heavy_thr = 128u;
+        heavy_valid = false;
+        serving = true;
+        inited = true;
+        return sync();
+    }
+
+    // PXD_FULL when the graph does not fit the free capacity right now
+    int serve_admit(uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges, uint64_t tag,
+                    int32_t* slot_out) {
+        if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
+        if (!slot_out || (!ranges) || (m && !edges)) return set_err(PXD_ERR_ARG, "null argument");
+        const uint64_t arcs = 2ull * m;
+        *slot_out = -1;
+        if (ranges[0] != 0 || ranges[n] != arcs)
+            return set_err(PXD_ERR_ARG, "ranges[n] must equal 2m and ranges[0] == 0");
+        uint32_t md = 0;
+        for (uint32_t u = 0; u < n; ++u) {
+            if (ranges[u + 1] < ranges[u]) return set_err(PXD_ERR_ARG, "ranges decrease at %u", u);
+            md = std::max(md, ranges[u + 1] - ranges[u]);
+        }
+        for (uint64_t j = 0; j < arcs; ++j)
+            if (edges[j] >= n)
+                return set_err(PXD_ERR_ARG, "arc %llu points to %u >= n", (unsigned long long)j, edges[j]);
+        if (n == 0) {                 // nothing to decompose: an empty partition right away
+            done.push_back(Done{tag, 0, 1, {}});
+            return PXD_OK;
+        }
+        if (n > N || arcs > A) return set_err(PXD_ERR_ARG, "graph larger than the serving capacity");
+        if (free_slots.empty()) return PXD_FULL;
+        uint64_t nb = 0, ab = 0;
+        if (!take_range(free_nodes, n, &nb)) return PXD_FULL;
+        if (!take_range(free_arcs, arcs, &ab)) {
+            give_range(free_nodes, nb, n);
+            return PXD_FULL;
+        }
+        const uint32_t g = free_slots.back();
+        free_slots.pop_back();
+        slots[g] = Slot{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, true};
+        HIPC(hipMemcpyAsync(d_stage, ranges, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
+        if (arcs) HIPC(hipMemcpyAsync(d_nbr + ab, edges, arcs * 4, hipMemcpyHostToDevice, st));
+        k_admit_nodes<<<grid_for(n, BS), BS, 0, st>>>((uint32_t)nb, n, (uint32_t)ab, g, nc, d_stage, P.seed, d_row,
+                                                      d_deg, d_dint, d_x[cur], d_y[cur], d_xkey, d_label, d_alive,
+                                                      d_stamp, d_ngraph, d_pnix[0] + na, d_pclu[0] + na,
+                                                      d_alist[acur] + nalist);
+        if (arcs) k_rebase<<<grid_for(arcs, BS), BS, 0, st>>>(arcs, d_nbr + ab, (uint32_t)nb);
+        k_admit_cluster<<<1, 64, 0, st>>>(nc, na, n, g, (uint32_t)nb, (uint32_t)arcs, iter1, ct[ctc].cols(), d_gsmall,
+                                          d_gsmall_next, d_gsurv, d_giter, d_gseen, d_gstart);
+        HIPC(hipGetLastError());
+        CHK(sync());                  // the caller's host buffers may go away after return
+        na += n;
+        nalist += n;
+        nc += 1;
+        heavy_valid = false;
+        sweep_done = false;
+        maxdeg = std::max(maxdeg, md);
+        live_graphs++;
+        live_nodes += n;
+        live_arcs += arcs;
+        *slot_out = (int32_t)g;
+        return PXD_OK;
+    }
+
+    // one iteration over every live graph; graphs that finished move to `done`
+    int serve_step(uint32_t* finished) {
+        if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
+        uint32_t nf = 0;
+        if (na > 0) {
+            const double avg = live_nodes ? (double)live_arcs / (double)live_nodes : 0.0;
+            gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
+            if (heavy_thr != 32u * gs) {
+                heavy_thr = 32u * gs;
+                heavy_valid = false;
+            }
+            CHK(walk());
+            CHK(sweep());
+            CHK(cut());
+            HIPC(hipMemcpyAsync(h_giter.data(), d_giter, G * 4, hipMemcpyDeviceToHost, st));
+            CHK(sync());
+            std::vector<int32_t> lab;
+            for (uint32_t g = 0; g < G; ++g) {
+                Slot& S = slots[g];
+                if (!S.used || h_giter[g] == 0) continue;
+                lab.resize(S.n);
+                HIPC(hipMemcpy(lab.data(), d_label + S.base, (size_t)S.n * 4, hipMemcpyDeviceToHost));
+                Done D{S.tag, 0, h_giter[g], std::vector<int32_t>(S.n)};
+                D.k = dense_rank(lab.data(), S.n, D.ids.data());
+                done.push_back(std::move(D));
+                give_range(free_nodes, S.base, S.n);
+                give_range(free_arcs, S.abase, S.arcs);
+                S.used = false;
+                free_slots.push_back(g);
+                live_graphs--;
+                live_nodes -= S.n;
+                live_arcs -= S.arcs;
+                ++nf;
+            }
+        }
+        if (finished) *finished = nf;
+        return PXD_OK;
+    }
+
     int run(int32_t* iterations) {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
+        if (serving) return set_err(PXD_ERR_ARG, "a serving engine advances with pxd_serve_step");
         HIPC(hipEventRecord(ev0, st));
         const uint32_t maxit = P.max_iter;
         uint32_t i = 0;
@@ -908,17 +1152,13 @@ static int require(pxd_engine* e) {
 int pxd_get_final_partition(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters) {
     CHK(require(e));
     if ((e->N && !ids_out) || !num_clusters) return set_err(PXD_ERR_ARG, "null output");
+    if (e->serving) return set_err(PXD_ERR_ARG, "a serving engine returns results with pxd_serve_take");
     std::vector<int32_t> lab(e->N);
     if (e->N) HIPC(hipMemcpy(lab.data(), e->d_label, e->N * 4, hipMemcpyDeviceToHost));
     // cuda/devPartition.h:798-823 per graph: sort, unique, rank
     for (uint32_t g = 0; g < e->G; ++g) {
         const uint32_t b = e->h_gnode[g], en = e->h_gnode[g + 1];
-        std::vector<int32_t> u(lab.begin() + b, lab.begin() + en);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        for (uint32_t i = b; i < en; ++i)
-            ids_out[i] = (int32_t)(std::lower_bound(u.begin(), u.end(), lab[i]) - u.begin());
-        num_clusters[g] = (int32_t)u.size();
+        num_clusters[g] = dense_rank(lab.data() + b, en - b, ids_out + b);
     }
     return 0;
 }
@@ -1099,6 +1339,59 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
     if (total_ms) *total_ms = e->walk_ms;
     if (alg_bytes) *alg_bytes = e->walk_bytes;
     return 0;
+}
+
+int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t arc_capacity, uint32_t graph_slots,
+                     pxd_engine** out) {
+    CHK(pxd_create(params, out));
+    const int rc = (*out)->serve_create(node_capacity, arc_capacity, graph_slots);
+    if (rc != 0) {
+        delete *out;
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
+                    uint64_t tag, int32_t* slot) {
+    return e ? e->serve_admit(n, m, ranges, edges, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+}
+
+int pxd_serve_step(pxd_engine* e, uint32_t* finished) {
+    return e ? e->serve_step(finished) : set_err(PXD_ERR_ARG, "null engine");
+}
+
+int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n) {
+    if (!e || !tag || !n) return set_err(PXD_ERR_ARG, "null argument");
+    if (e->done.empty()) return PXD_EMPTY;
+    *tag = e->done.front().tag;
+    *n = (uint32_t)e->done.front().ids.size();
+    return PXD_OK;
+}
+
+int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations) {
+    if (!e || !num_clusters || !iterations) return set_err(PXD_ERR_ARG, "null argument");
+    if (e->done.empty()) return PXD_EMPTY;
+    pxd_engine::Done& D = e->done.front();
+    if (!D.ids.empty()) {
+        if (!ids_out) return set_err(PXD_ERR_ARG, "null output");
+        std::memcpy(ids_out, D.ids.data(), D.ids.size() * 4);
+    }
+    *num_clusters = D.k;
+    *iterations = D.it;
+    e->done.pop_front();
+    return PXD_OK;
+}
+
+int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
+                    uint32_t* active_nodes, uint32_t* waiting_results) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    if (live_graphs) *live_graphs = e->live_graphs;
+    if (live_nodes) *live_nodes = e->live_nodes;
+    if (live_arcs) *live_arcs = e->live_arcs;
+    if (active_nodes) *active_nodes = e->na;
+    if (waiting_results) *waiting_results = (uint32_t)e->done.size();
+    return PXD_OK;
 }
 
 }  // extern "C"

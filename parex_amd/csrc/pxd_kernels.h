@@ -60,6 +60,12 @@ __device__ float init_value(uint32_t idx, uint32_t deg, uint32_t seed) {
     return (sigma * S3) * e;
 }
 
+// columns of a cluster table (device pointers)
+struct ClusterCols {
+    uint32_t *cs, *cgraph, *cmin, *cflag, *cvol;
+    int32_t* clabel;
+};
+
 __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict__ gnode,
                              const uint32_t* __restrict__ row, uint32_t seed,
                              uint32_t* __restrict__ deg, uint32_t* __restrict__ dint,
@@ -67,7 +73,8 @@ __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict_
                              uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
                              uint8_t* __restrict__ alive, uint32_t* __restrict__ stamp,
                              uint32_t* __restrict__ pnix, uint32_t* __restrict__ alist,
-                             uint32_t* __restrict__ pclu, const uint32_t* __restrict__ gclu) {
+                             uint32_t* __restrict__ pclu, const uint32_t* __restrict__ gclu,
+                             uint32_t* __restrict__ ngraph) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = G;                  // graph of u: last g with gnode[g] <= u
         while (hi - lo > 1) {
@@ -87,6 +94,66 @@ __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict_
         pnix[u] = u;
         alist[u] = u;
         pclu[u] = gclu[lo];
+        ngraph[u] = lo;
+    }
+}
+
+// One admitted graph (serving): nodes [base, base + n) with raw ranges rg
+// (rg[0] = 0) and arcs at abase.  Same initial state as k_init_nodes gives a
+// graph of a batch; its nodes are appended to the positions (cluster c, in
+// node order) and to the active list.
+__global__ void k_admit_nodes(uint32_t base, uint32_t n, uint32_t abase, uint32_t slot, uint32_t c,
+                              const uint32_t* __restrict__ rg, uint32_t seed, uint32_t* __restrict__ row,
+                              uint32_t* __restrict__ deg, uint32_t* __restrict__ dint, double* __restrict__ x,
+                              double* __restrict__ y, uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
+                              uint8_t* __restrict__ alive, uint32_t* __restrict__ stamp,
+                              uint32_t* __restrict__ ngraph, uint32_t* __restrict__ pnix,
+                              uint32_t* __restrict__ pclu, uint32_t* __restrict__ alist) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t u = base + i;
+        const uint32_t d = rg[i + 1] - rg[i];
+        row[u] = abase + rg[i];
+        deg[u] = d;
+        dint[u] = d;
+        const double v = (double)init_value(i, d, seed);
+        x[u] = v;
+        y[u] = d ? v / (double)d : 0.0;
+        xkey[u] = ord_f32((float)v);
+        label[u] = 0;
+        alive[u] = 1;
+        stamp[u] = 0;
+        ngraph[u] = slot;
+        pnix[i] = u;
+        pclu[i] = c;
+        alist[i] = u;
+    }
+}
+
+// arcs of an admitted graph: local -> global node ids
+__global__ void k_rebase(uint64_t n, uint32_t* __restrict__ nbr, uint32_t base) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+        nbr[j] += base;
+}
+
+// the admitted graph's single cluster, appended as cluster c at positions
+// [na, na + n), and its per-graph state
+__global__ void k_admit_cluster(uint32_t c, uint32_t na, uint32_t n, uint32_t slot, uint32_t base, uint32_t vol,
+                                uint32_t iter, ClusterCols T, int32_t* gsmall, int32_t* gsmall_next,
+                                uint32_t* gsurv, int32_t* giter, uint32_t* gseen, uint32_t* gstart) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        T.cs[c] = na;
+        T.cs[c + 1] = na + n;
+        T.cgraph[c] = slot;
+        T.cmin[c] = base;
+        T.cflag[c] = 0u;
+        T.clabel[c] = 0;
+        T.cvol[c] = vol;
+        gsmall[slot] = 0;
+        gsmall_next[slot] = 0;
+        gsurv[slot] = 0u;
+        giter[slot] = 0;
+        gseen[slot] = 0u;
+        gstart[slot] = iter;
     }
 }
 
@@ -920,7 +987,7 @@ __global__ void k_run_flags(uint32_t na, const uint32_t* __restrict__ key, uint3
 }
 
 // new cluster table from runs of equal regroup keys (sorted by minimum node)
-__global__ void k_build_clusters(uint32_t na, uint32_t G, const uint32_t* __restrict__ gnode,
+__global__ void k_build_clusters(uint32_t na, const uint32_t* __restrict__ ngraph,
                                  const uint32_t* __restrict__ key, const uint32_t* __restrict__ f,
                                  const uint32_t* __restrict__ fs, uint32_t* __restrict__ cs,
                                  uint32_t* __restrict__ cmin, uint32_t* __restrict__ cgraph,
@@ -930,14 +997,9 @@ __global__ void k_build_clusters(uint32_t na, uint32_t G, const uint32_t* __rest
         pclu[k] = c;
         if (f[k]) {
             const uint32_t r = key[k];
-            uint32_t lo = 0, hi = G;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (gnode[mid] <= r) lo = mid; else hi = mid;
-            }
             cs[c] = k;
             cmin[c] = r;
-            cgraph[c] = lo;
+            cgraph[c] = ngraph[r];
             cflag[c] = CF_CONN;
         }
         if (k == na - 1) cs[c + 1] = na;
@@ -1042,13 +1104,14 @@ __global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const
     }
 }
 
+// iteration counts relative to the iteration a graph joined at (gstart)
 __global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive, int32_t* __restrict__ gsmall,
                                int32_t* __restrict__ gsmall_next, uint32_t* __restrict__ gsurv,
-                               int32_t* __restrict__ giter, uint32_t iter1) {
+                               int32_t* __restrict__ giter, const uint32_t* __restrict__ gstart, uint32_t iter1) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
         if (gactive[g]) {
             gsmall[g] = gsmall_next[g];
-            if (gsurv[g] == 0u && giter[g] == 0) giter[g] = (int32_t)iter1 + 1;
+            if (gsurv[g] == 0u && giter[g] == 0) giter[g] = (int32_t)(iter1 - gstart[g]) + 1;
         }
         gsmall_next[g] = 0;
         gsurv[g] = 0u;
@@ -1099,11 +1162,6 @@ __global__ void k_pos_compact(uint32_t na, const uint32_t* __restrict__ f, const
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
         if (f[k]) { pnix_o[fs[k]] = pnix[k]; pclu_o[fs[k]] = cscan[pclu[k]]; }
 }
-
-struct ClusterCols {
-    uint32_t *cs, *cgraph, *cmin, *cflag, *cvol;
-    int32_t* clabel;
-};
 
 __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __restrict__ ckeep,
                               const uint32_t* __restrict__ cscan, const uint32_t* __restrict__ pscan,
