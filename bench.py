@@ -61,6 +61,115 @@ def workload(name, graphs_limit=None):
     raise SystemExit(f"unknown workload {name}")
 
 
+def serve_pool(seed=13):
+    """Request graphs of the serving mix: SBM with 10K-200K nodes (n log-uniform,
+    other parameters as the 512-graph batch), tori with side 64-1024, RMAT with
+    scale 14-21 and edge factor 16.  Requests draw from a pool of distinct graphs
+    (generating 12,000 fresh inputs at the offered rate is not possible on the host)."""
+    rng = np.random.default_rng(seed)
+    specs = px.sbm_specs(48, 10000, 200000, 4, 64, (8.0, 32.0), (0.5, 3.0), seed)
+    sbm = px.sbm_batch(specs)
+    tor = [px.torus(int(L), int(round(0.005 * 2 * L * L)), seed + i)
+           for i, L in enumerate(rng.integers(64, 1025, 16))]
+    rm = [px.rmat(int(sc), 16, 0.57, 0.19, 0.19, seed + i) for i, sc in enumerate(range(14, 22))]
+    return {"sbm": sbm, "torus": tor, "rmat": rm}
+
+
+def serve_workload(a):
+    rng = np.random.default_rng(13)
+    t = time.perf_counter()
+    pool = serve_pool()
+    log(f"serving pool: {sum(len(v) for v in pool.values())} graphs ({time.perf_counter() - t:.1f} s)")
+    rate, dur = 200.0, a.duration
+    gaps = rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 100)
+    arr = np.cumsum(gaps)
+    arr = arr[arr < dur]
+    kinds = rng.choice(["sbm", "torus", "rmat"], size=len(arr), p=[0.6, 0.3, 0.1])
+    req = [pool[k][int(rng.integers(len(pool[k])))] for k in kinds]
+    offered = sum(g.m for g in req) / dur
+    log(f"{len(req)} requests over {dur:.0f} s, offered {offered:.3e} edges/s")
+    srv = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=4096,
+                    params=px.Params(record_node_data=False))
+    # untimed warm-up: one graph of each kind
+    for i, k in enumerate(["sbm", "torus", "rmat"]):
+        srv.admit(pool[k][0], 10 ** 9 + i)
+    while srv.stats()["live_graphs"]:
+        srv.step()
+    srv.results()
+    from collections import deque
+    queue, lat, done_edges = deque(), np.zeros(len(req)), 0
+    finished, i, steps, inflight_sum = 0, 0, 0, 0
+    t0 = time.perf_counter()
+    last_log = t0
+    while finished < len(req):
+        now = time.perf_counter() - t0
+        while i < len(req) and arr[i] <= now:
+            queue.append(i)
+            i += 1
+        while queue and srv.admit(req[queue[0]], queue[0]):
+            queue.popleft()
+        live = srv.stats()["live_graphs"]
+        if live:
+            srv.step()
+            steps += 1
+            inflight_sum += live
+            tc = time.perf_counter() - t0
+            for tag, ids, k, it in srv.results():
+                lat[tag] = tc - arr[tag]
+                done_edges += req[tag].m
+                finished += 1
+        elif i < len(req):
+            time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"  t={last_log - t0:.0f} s: {finished}/{len(req)} done, {live} in flight, {len(queue)} queued")
+    wall = time.perf_counter() - t0
+    # results in the mix equal solo results (sample, untimed)
+    same = True
+    probe = rng.choice(len(req), size=min(12, len(req)), replace=False)
+    srv2 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=64,
+                     params=px.Params(record_node_data=False))
+    ref = {}
+    for j in probe:
+        srv2.admit(req[j], int(j))
+        while srv2.stats()["live_graphs"]:
+            srv2.step()
+        for tag, ids, k, it in srv2.results():
+            ref[tag] = (ids, k, it)
+    srv2.close()
+    # replay the probes inside a fresh mix to compare (the timed run does not keep ids)
+    srv3 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=256,
+                     params=px.Params(record_node_data=False))
+    mixset = list(probe) + list(rng.choice(len(req), size=24, replace=False))
+    for j, tagj in enumerate(mixset):
+        srv3.admit(req[tagj], (j << 32) | int(tagj))
+        srv3.step()
+    while srv3.stats()["live_graphs"]:
+        srv3.step()
+    for tag, ids, k, it in srv3.results():
+        tj = tag & 0xFFFFFFFF
+        if tj in ref and (tag >> 32) < len(probe):
+            rid, rk, rit = ref[tj]
+            same = same and rk == k and rit == it and np.array_equal(rid, ids)
+    srv3.close()
+    srv.close()
+    p50, p99 = np.percentile(lat * 1e3, [50, 99])
+    value = done_edges / wall
+    line = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": steps, "warmup": 3,
+        "ms_per_step": wall * 1e3 / max(steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serving: Poisson 200 req/s, 60% SBM 10K-200K nodes, 30% torus 64^2-1024^2, "
+                               "10% RMAT scale 14-21 ef 16; requests drawn from a pool of 72 graphs",
+                   "requests": len(req), "duration_s": dur, "seed": 13, "offered_edges_per_s": offered},
+        "p50_ms": p50, "p99_ms": p99, "mean_in_flight": inflight_sum / max(steps, 1),
+        "wall_s": wall, "results_equal_solo": same,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def profiled_traffic(workload):
     """HBM bytes per k_walk launch from the committed rocprofv3 summary of
     this workload (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated)."""
@@ -121,11 +230,14 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=60.0, help="serving workload: seconds of arrivals")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         return 2
+    if a.workload == "c5":
+        return serve_workload(a)
 
     t = time.perf_counter()
     graphs, cfg = workload(a.workload, a.graphs)

@@ -1038,7 +1038,8 @@ struct pxd_engine {
 };
 
 __global__ void k_mark_seen(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gseen) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) gseen[cgraph[c]] = 1u;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        if (c == 0 || cgraph[c] != cgraph[c - 1]) gseen[cgraph[c]] = 1u;    // first cluster of each graph
 }
 
 __global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32_t* __restrict__ dint,

@@ -798,16 +798,27 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
 __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbest, double thr,
                          uint32_t* __restrict__ cflag, uint32_t* __restrict__ coff,
                          float* __restrict__ csp, Counters* cnt) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const unsigned long long b = cbest[c];
-        const float sp = unord_f32((uint32_t)(b >> 32));
-        const uint32_t off = 0xFFFFFFFFu - (uint32_t)b;
-        csp[c] = sp;
-        coff[c] = off;
-        uint32_t f = cflag[c] & CF_CONN;
-        if ((double)sp < thr) { f |= CF_CUT; atomicAdd(&cnt->ncut, 1u); }
-        if ((f & CF_CUT) || !(f & CF_CONN)) { f |= CF_CC; atomicAdd(&cnt->ncc, 1u); }
-        cflag[c] = f;
+    for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += gridDim.x * blockDim.x) {
+        const uint32_t c = c0 + threadIdx.x;
+        uint32_t f = 0u;
+        if (c < nc) {
+            const unsigned long long b = cbest[c];
+            const float sp = unord_f32((uint32_t)(b >> 32));
+            const uint32_t off = 0xFFFFFFFFu - (uint32_t)b;
+            csp[c] = sp;
+            coff[c] = off;
+            f = cflag[c] & CF_CONN;
+            if ((double)sp < thr) f |= CF_CUT;
+            if ((f & CF_CUT) || !(f & CF_CONN)) f |= CF_CC;
+            cflag[c] = f;
+        }
+        // counts: one atomic per wave
+        const uint32_t ncut = (uint32_t)__popcll(__ballot(c < nc && (f & CF_CUT)));
+        const uint32_t ncc = (uint32_t)__popcll(__ballot(c < nc && (f & CF_CC)));
+        if (lane_id() == 0) {
+            if (ncut) atomicAdd(&cnt->ncut, ncut);
+            if (ncc) atomicAdd(&cnt->ncc, ncc);
+        }
     }
 }
 
@@ -1090,16 +1101,37 @@ __global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const
                          const uint32_t* __restrict__ ckeep, const uint32_t* __restrict__ kscan,
                          const int32_t* __restrict__ gsmall, int32_t* __restrict__ gsmall_next,
                          uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const uint32_t g = cgraph[c];
-        const uint32_t f = gfirst[g];
-        if (ckeep[c]) {
-            clabel[c] = (int32_t)(kscan[c] - kscan[f]);
-            atomicAdd(&gsurv[g], 1u);
-        } else {
-            const int32_t lab = gsmall[g] - (int32_t)(c - f) - 1;
-            clabel[c] = lab;
-            atomicMin(&gsmall_next[g], lab);
+    // a graph's clusters are contiguous: the per-graph survivor count and
+    // smallest new label are reduced over the wave first (one atomic per graph
+    // run in the wave instead of one per cluster on a single address)
+    for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += gridDim.x * blockDim.x) {
+        const uint32_t c = c0 + threadIdx.x;
+        const bool ok = c < nc;
+        const uint32_t g = ok ? cgraph[c] : 0xFFFFFFFFu;
+        uint32_t surv = 0u;
+        int32_t lab = 0x7FFFFFFF;
+        if (ok) {
+            const uint32_t f = gfirst[g];
+            if (ckeep[c]) {
+                clabel[c] = (int32_t)(kscan[c] - kscan[f]);
+                surv = 1u;
+            } else {
+                lab = gsmall[g] - (int32_t)(c - f) - 1;
+                clabel[c] = lab;
+            }
+        }
+        const uint32_t g0 = __shfl(g, 0);
+        if (__all(g == g0 || g == 0xFFFFFFFFu)) {
+            if (g0 == 0xFFFFFFFFu) continue;
+            surv = group_sum<WAVE>(surv);
+            for (int off = WAVE / 2; off > 0; off >>= 1) lab = min(lab, __shfl_xor(lab, off));
+            if (lane_id() == 0) {
+                if (surv) atomicAdd(&gsurv[g0], surv);
+                if (lab != 0x7FFFFFFF) atomicMin(&gsmall_next[g0], lab);
+            }
+        } else if (ok) {
+            if (surv) atomicAdd(&gsurv[g], 1u);
+            if (lab != 0x7FFFFFFF) atomicMin(&gsmall_next[g], lab);
         }
     }
 }
