@@ -562,12 +562,13 @@ struct pxd_engine {
     }
     template <int GSZ>
     void launch_link(const uint32_t* list, uint32_t nl) {
-        k_cc_link<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root, d_cnt);
+        k_cc_link<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root,
+                                                                heavy_thr, d_cnt);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
         k_disable<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
-                                                                d_x[cur], d_y[cur]);
+                                                                d_x[cur], d_y[cur], heavy_thr);
     }
 
     // chunk map of the active rows above heavy_thr (valid until the next cut)
@@ -755,6 +756,14 @@ struct pxd_engine {
         nc = nc_new;
         if (nl) {
             PXD_DISPATCH_GS(launch_disable, nl);
+            if (heavy_thr != 0xFFFFFFFFu) {
+                heavy_valid = false;
+                k_heavy_flags<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_alive, d_dint, heavy_thr, d_hcnt);
+                CHK(scan(d_hcnt, d_hoff, nl));
+                k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_hcnt, d_hoff, d_cclist, d_hrow);
+                k_disable_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nl, d_hrow, d_row, d_dint, d_nbr, d_root,
+                                                              d_x[cur], d_y[cur]);
+            }
             HIPC(hipGetLastError());
         }
         if (nc) {
@@ -788,9 +797,20 @@ struct pxd_engine {
         if (nu) {
             k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
             HIPC(hipGetLastError());
+            const bool heavy = heavy_thr != 0xFFFFFFFFu;
+            if (heavy) {              // chunk map of the list's long rows (the walk's map is not needed now)
+                heavy_valid = false;
+                k_heavy_count<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_cclist2, d_dint, heavy_thr, d_hcnt);
+                CHK(scan(d_hcnt, d_hoff, nu));
+                k_heavy_expand<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_hcnt, d_hoff, d_hrow);
+                HIPC(hipGetLastError());
+            }
             for (int round = 0;; ++round) {
                 HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
                 PXD_DISPATCH_GS(launch_link, d_cclist2, nu);
+                if (heavy)
+                    k_cc_link_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nu, d_hrow, d_hoff, d_cclist2, d_row, d_dint,
+                                                                  d_nbr, d_part, d_root, d_cnt);
                 HIPC(hipGetLastError());
                 CHK(read_counters());
                 if (!hc.link_changed) break;
@@ -920,14 +940,6 @@ struct pxd_engine {
         *slot_out = -1;
         if (ranges[0] != 0 || ranges[n] != arcs)
             return set_err(PXD_ERR_ARG, "ranges[n] must equal 2m and ranges[0] == 0");
-        uint32_t md = 0;
-        for (uint32_t u = 0; u < n; ++u) {
-            if (ranges[u + 1] < ranges[u]) return set_err(PXD_ERR_ARG, "ranges decrease at %u", u);
-            md = std::max(md, ranges[u + 1] - ranges[u]);
-        }
-        for (uint64_t j = 0; j < arcs; ++j)
-            if (edges[j] >= n)
-                return set_err(PXD_ERR_ARG, "arc %llu points to %u >= n", (unsigned long long)j, edges[j]);
         if (n == 0) {                 // nothing to decompose: an empty partition right away
             done.push_back(Done{tag, 0, 1, {}});
             return PXD_OK;
@@ -940,11 +952,23 @@ struct pxd_engine {
             give_range(free_nodes, nb, n);
             return PXD_FULL;
         }
+        // upload and check on the device before anything else reads the graph
+        HIPC(hipMemcpyAsync(d_stage, ranges, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
+        if (arcs) HIPC(hipMemcpyAsync(d_nbr + ab, edges, arcs * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemsetAsync(&d_cnt->admit_bad, 0, 4, st));
+        k_admit_check<<<grid_for(std::max<uint64_t>(n, arcs), BS), BS, 0, st>>>(n, d_stage, arcs, d_nbr + ab,
+                                                                              &d_cnt->admit_bad);
+        HIPC(hipGetLastError());
+        uint32_t bad = 0;
+        CHK(read_u32(&d_cnt->admit_bad, &bad));   // also: the caller's buffers are copied
+        if (bad) {
+            give_range(free_nodes, nb, n);
+            give_range(free_arcs, ab, arcs);
+            return set_err(PXD_ERR_ARG, (bad & 1u) ? "ranges decrease" : "an arc points to a node >= n");
+        }
         const uint32_t g = free_slots.back();
         free_slots.pop_back();
         slots[g] = Slot{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, true};
-        HIPC(hipMemcpyAsync(d_stage, ranges, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
-        if (arcs) HIPC(hipMemcpyAsync(d_nbr + ab, edges, arcs * 4, hipMemcpyHostToDevice, st));
         k_admit_nodes<<<grid_for(n, BS), BS, 0, st>>>((uint32_t)nb, n, (uint32_t)ab, g, nc, d_stage, P.seed, d_row,
                                                       d_deg, d_dint, d_x[cur], d_y[cur], d_xkey, d_label, d_alive,
                                                       d_stamp, d_ngraph, d_pnix[0] + na, d_pclu[0] + na,
@@ -953,13 +977,11 @@ struct pxd_engine {
         k_admit_cluster<<<1, 64, 0, st>>>(nc, na, n, g, (uint32_t)nb, (uint32_t)arcs, iter1, ct[ctc].cols(), d_gsmall,
                                           d_gsmall_next, d_gsurv, d_giter, d_gseen, d_gstart);
         HIPC(hipGetLastError());
-        CHK(sync());                  // the caller's host buffers may go away after return
         na += n;
         nalist += n;
         nc += 1;
         heavy_valid = false;
         sweep_done = false;
-        maxdeg = std::max(maxdeg, md);
         live_graphs++;
         live_nodes += n;
         live_arcs += arcs;

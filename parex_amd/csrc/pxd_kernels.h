@@ -33,6 +33,7 @@ struct Counters {
     uint32_t ncut, ncc;
     uint32_t cclist_n;
     uint32_t link_changed;
+    uint32_t admit_bad;
     uint32_t pad[7];
 };
 
@@ -127,6 +128,18 @@ __global__ void k_admit_nodes(uint32_t base, uint32_t n, uint32_t abase, uint32_
         pclu[i] = c;
         alist[i] = u;
     }
+}
+
+// checks of an uploaded graph before it joins: ranges non-decreasing (rg[0] =
+// 0 and rg[n] = 2m are checked on the host) and every arc below n
+__global__ void k_admit_check(uint32_t n, const uint32_t* __restrict__ rg, uint64_t arcs,
+                              const uint32_t* __restrict__ nbr, uint32_t* __restrict__ bad) {
+    uint32_t b = 0u;
+    const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = t0; i < n; i += step) b |= rg[i + 1] < rg[i] ? 1u : 0u;
+    for (uint64_t j = t0; j < arcs; j += step) b |= nbr[j] >= n ? 2u : 0u;
+    b = group_or<WAVE>(b);
+    if (b && lane_id() == 0) atomicOr(bad, b);
 }
 
 // arcs of an admitted graph: local -> global node ids
@@ -933,13 +946,14 @@ __global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ part,
-                                                uint32_t* __restrict__ root, Counters* cnt) {
+                                                uint32_t* __restrict__ root, uint32_t heavy, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
     uint32_t changed = 0;
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
         const uint32_t start = row[u], d = dint[u];
+        if (d > heavy) continue;      // k_cc_link_heavy takes rows above the threshold
         const uint32_t pp = part[u];
         uint32_t ru = cc_find(root, u);
         for (uint32_t j = lane; j < d; j += GS) {
@@ -951,6 +965,44 @@ __global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __r
             if (ld_dev(&root[hi]) > lo && atomicMin(&root[hi], lo) > lo) {
                 changed = 1;
                 if (hi == ru) ru = lo;    // u's tree now hangs below lo
+            }
+        }
+    }
+    if (__any(changed) && lane_id() == 0 && __atomic_load_n(&cnt->link_changed, __ATOMIC_RELAXED) == 0u)
+        atomicOr(&cnt->link_changed, 1u);
+}
+
+// the link rounds for rows above the threshold: one workgroup per chunk of
+// HEAVY_CHUNK arcs (chunk map over the link list)
+__global__ void __launch_bounds__(BS) k_cc_link_heavy(const uint32_t* __restrict__ nchunks,
+                                                      const uint32_t* __restrict__ hrow,
+                                                      const uint32_t* __restrict__ hoff,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ row,
+                                                      const uint32_t* __restrict__ dint,
+                                                      const uint32_t* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ part,
+                                                      uint32_t* __restrict__ root, Counters* cnt) {
+    const uint32_t H = *nchunks;
+    uint32_t changed = 0;
+    for (uint32_t b = blockIdx.x; b < H; b += gridDim.x) {
+        const uint32_t i = hrow[b];
+        const uint32_t u = list[i];
+        const uint32_t d = dint[u], pp = part[u];
+        const uint32_t* p = nbr + row[u];
+        const uint32_t j0 = (b - hoff[i]) * HEAVY_CHUNK;
+        uint32_t ru = cc_find(root, u);
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t j = j0 + q * BS + threadIdx.x;
+            if (j >= d) break;
+            const uint32_t v = p[j];
+            if (part[v] != pp) continue;
+            const uint32_t rv = cc_find(root, v);
+            if (rv == ru) continue;
+            const uint32_t hi = max(ru, rv), lo = min(ru, rv);
+            if (ld_dev(&root[hi]) > lo && atomicMin(&root[hi], lo) > lo) {
+                changed = 1;
+                if (hi == ru) ru = lo;
             }
         }
     }
@@ -1234,13 +1286,15 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ row,
                                                 uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ root,
-                                                const double* __restrict__ x, double* __restrict__ y) {
+                                                const double* __restrict__ x, double* __restrict__ y,
+                                                uint32_t heavy) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
         if (!alive[u]) continue;                      // uniform within the group
         const uint32_t start = row[u], d = dint[u];
+        if (d > heavy) continue;                      // k_disable_heavy
         const uint32_t ru = root[u];
         uint32_t kept = 0;
         // four chunks of GS arcs loaded (indices, then roots) before the
@@ -1266,6 +1320,57 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
             dint[u] = kept;
             y[u] = kept ? x[u] / (double)kept : 0.0;
         }
+    }
+}
+
+// 1 for the alive rows of the list above the threshold
+__global__ void k_heavy_flags(uint32_t n, const uint32_t* __restrict__ list, const uint8_t* __restrict__ alive,
+                              const uint32_t* __restrict__ dint, uint32_t heavy, uint32_t* __restrict__ f) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        f[i] = (alive[u] && dint[u] > heavy) ? 1u : 0u;
+    }
+}
+
+// arc disabling for rows above the threshold: one workgroup per row, chunks of
+// HEAVY_CHUNK arcs in order; each thread owns four consecutive arcs, a block
+// scan ranks the kept ones, and every write (after the barrier that ends the
+// chunk's loads) goes to an index no larger than the arc's own
+__global__ void __launch_bounds__(BS) k_disable_heavy(const uint32_t* __restrict__ nrows,
+                                                      const uint32_t* __restrict__ hnode,
+                                                      const uint32_t* __restrict__ row,
+                                                      uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ root,
+                                                      const double* __restrict__ x, double* __restrict__ y) {
+    __shared__ uint32_t lds[BS / WAVE];
+    const uint32_t R = *nrows;
+    for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
+        const uint32_t u = hnode[r];
+        const uint32_t start = row[u], d = dint[u], ru = root[u];
+        uint32_t kept = 0;
+        for (uint32_t j0 = 0; j0 < d; j0 += HEAVY_CHUNK) {
+            uint32_t v[4], nk = 0;
+            bool k[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint32_t j = j0 + 4u * threadIdx.x + q;
+                v[q] = nbr[start + min(j, d - 1u)];
+                k[q] = j < d && root[v[q]] == ru;
+                nk += k[q] ? 1u : 0u;
+            }
+            uint32_t tot;
+            uint32_t w = kept + block_excl_scan<BS>(nk, lds, tot);
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q)
+                if (k[q]) nbr[start + w++] = v[q];
+            kept += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && kept != d) {
+            dint[u] = kept;
+            y[u] = kept ? x[u] / (double)kept : 0.0;
+        }
+        __syncthreads();
     }
 }
 

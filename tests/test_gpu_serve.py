@@ -112,3 +112,24 @@ def test_serve_rejects_batch_calls_and_oversized_graphs():
     res = srv.results()
     assert len(res) == 1 and res[0][0] == 5 and res[0][2] == 0 and res[0][3] == 1
     srv.close()
+
+
+def test_serve_refuses_invalid_graph_and_carries_on():
+    """Arcs outside the graph are caught on the device before the graph joins;
+    its room is returned and the server goes on serving."""
+    good = px.sbm(300, 3, 0.1, 0.01, 4)
+    edges = good.edges.copy()
+    edges[17] = good.n                                   # one arc past the last node
+    bad = px.Graph(good.n, good.m, good.ranges.copy(), edges)
+    ranges = good.ranges.copy()
+    ranges[5], ranges[6] = ranges[6], ranges[5]          # decreasing ranges
+    bad2 = px.Graph(good.n, good.m, ranges, good.edges.copy())
+    srv = px.Server(node_capacity=400, arc_capacity=good.edges.size, graph_slots=1)
+    for g in (bad, bad2):
+        with pytest.raises(px.PxdError):
+            srv.admit(g, 1)
+    assert srv.stats()["live_graphs"] == 0
+    got = {}
+    drain(srv, [0], [good], got)
+    srv.close()
+    check([good], got)
