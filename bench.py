@@ -75,10 +75,29 @@ def serve_pool(seed=13):
     return {"sbm": sbm, "torus": tor, "rmat": rm}
 
 
+def pinned_pool(pool):
+    """Copies of the request graphs in pinned host memory, as a server's
+    receive buffers would be: uploads are then plain DMA."""
+    import torch
+    keep, out = [], {}
+    for kind, graphs in pool.items():
+        out[kind] = []
+        for g in graphs:
+            arrs = []
+            for a in (g.ranges, g.edges):
+                t = torch.empty(max(a.size, 1), dtype=torch.int32, pin_memory=True)
+                keep.append(t)
+                v = t.numpy().view(np.uint32)[:a.size]
+                v[:] = a
+                arrs.append(v)
+            out[kind].append(px.Graph(g.n, g.m, arrs[0], arrs[1]))
+    return out, keep
+
+
 def serve_workload(a):
     rng = np.random.default_rng(13)
     t = time.perf_counter()
-    pool = serve_pool()
+    pool, _pinned = pinned_pool(serve_pool())
     log(f"serving pool: {sum(len(v) for v in pool.values())} graphs ({time.perf_counter() - t:.1f} s)")
     rate, dur = 200.0, a.duration
     gaps = rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 100)
@@ -95,10 +114,11 @@ def serve_workload(a):
         srv.admit(pool[k][0], 10 ** 9 + i)
     while srv.stats()["live_graphs"]:
         srv.step()
-    srv.results()
+    srv.results(wait=True)
     from collections import deque
     queue, lat, done_edges = deque(), np.zeros(len(req)), 0
     finished, i, steps, inflight_sum = 0, 0, 0, 0
+    tsp = {"admit": 0.0, "step": 0.0, "results": 0.0, "stats": 0.0, "idle": 0.0}
     t0 = time.perf_counter()
     last_log = t0
     while finished < len(req):
@@ -106,20 +126,33 @@ def serve_workload(a):
         while i < len(req) and arr[i] <= now:
             queue.append(i)
             i += 1
+        ta = time.perf_counter()
         while queue and srv.admit(req[queue[0]], queue[0]):
             queue.popleft()
+        tb = time.perf_counter()
         live = srv.stats()["live_graphs"]
+        tc0 = time.perf_counter()
+        tsp["admit"] += tb - ta
+        tsp["stats"] += tc0 - tb
         if live:
             srv.step()
             steps += 1
             inflight_sum += live
             tc = time.perf_counter() - t0
+            tsp["step"] += tc + t0 - tc0
             for tag, ids, k, it in srv.results():
-                lat[tag] = tc - arr[tag]
+                lat[tag] = time.perf_counter() - t0 - arr[tag]     # when the client has the result
                 done_edges += req[tag].m
                 finished += 1
+            tsp["results"] += time.perf_counter() - t0 - tc
         elif i < len(req):
             time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+            tsp["idle"] += time.perf_counter() - tc0
+        elif not queue:               # nothing in flight: collect the last results
+            for tag, ids, k, it in srv.results(wait=True):
+                lat[tag] = time.perf_counter() - t0 - arr[tag]
+                done_edges += req[tag].m
+                finished += 1
         if time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
             log(f"  t={last_log - t0:.0f} s: {finished}/{len(req)} done, {live} in flight, {len(queue)} queued")
@@ -134,7 +167,7 @@ def serve_workload(a):
         srv2.admit(req[j], int(j))
         while srv2.stats()["live_graphs"]:
             srv2.step()
-        for tag, ids, k, it in srv2.results():
+        for tag, ids, k, it in srv2.results(wait=True):
             ref[tag] = (ids, k, it)
     srv2.close()
     # replay the probes inside a fresh mix to compare (the timed run does not keep ids)
@@ -146,7 +179,7 @@ def serve_workload(a):
         srv3.step()
     while srv3.stats()["live_graphs"]:
         srv3.step()
-    for tag, ids, k, it in srv3.results():
+    for tag, ids, k, it in srv3.results(wait=True):
         tj = tag & 0xFFFFFFFF
         if tj in ref and (tag >> 32) < len(probe):
             rid, rk, rit = ref[tj]
@@ -164,6 +197,7 @@ def serve_workload(a):
                    "requests": len(req), "duration_s": dur, "seed": 13, "offered_edges_per_s": offered},
         "p50_ms": p50, "p99_ms": p99, "mean_in_flight": inflight_sum / max(steps, 1),
         "wall_s": wall, "results_equal_solo": same,
+        "host_seconds": {k: round(v, 3) for k, v in tsp.items()},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -159,17 +159,24 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
 int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t arc_capacity, uint32_t graph_slots,
                      pxd_engine** out);
 /* Admit one graph (CSR as for pxd_initialize) under the caller's tag.
- * PXD_FULL: it does not fit now (retry after steps free room); the host
- * buffers may be reused once the call returns. */
+ * PXD_FULL: it does not fit now (retry after steps free room).  The uploads
+ * are queued (pinned host buffers make them plain DMA): the buffers must stay
+ * valid until the next pxd_serve_step returns.  The graph is checked on the
+ * device (non-decreasing ranges, arcs below n) and joins at that step; one
+ * that fails comes back as a result with num_clusters = -1. */
 int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
                     uint64_t tag, int32_t* slot);
 /* One iteration (walk, sweep, cut) of every graph in flight; *finished =
  * graphs that completed in it (their results wait for pxd_serve_take). */
 int pxd_serve_step(pxd_engine* e, uint32_t* finished);
-/* Tag and node count of the oldest waiting result; PXD_EMPTY if none. */
+/* Tag and node count of the oldest result; PXD_EMPTY if none is ready (the
+ * dense ranking runs on a worker thread of the engine). */
 int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n);
+/* Block until every finished result is ready. */
+int pxd_serve_wait(pxd_engine* e);
 /* Pop that result: cluster id per node (ids_out[n]), cluster count and the
- * iteration count its solo run reports (cuda/device.h:343-350). */
+ * iteration count its solo run reports (cuda/device.h:343-350), the ids
+ * densely ranked as cuda/devPartition.h:798-823 does. */
 int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations);
 int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
                     uint32_t* active_nodes, uint32_t* waiting_results);

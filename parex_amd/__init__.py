@@ -137,6 +137,7 @@ def pxd() -> C.CDLL:
         lib.pxd_serve_step.argtypes = [P, C.POINTER(C.c_uint32)]
         lib.pxd_serve_next.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
         lib.pxd_serve_take.argtypes = [P, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        lib.pxd_serve_wait.argtypes = [P]
         lib.pxd_serve_stats.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib._typed = True
@@ -152,7 +153,8 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
-    "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_take", "pxd_serve_stats",
+    "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
+    "pxd_serve_stats",
 ]
 
 PXD_FULL = 1       # pxd_serve_admit: no room right now
@@ -318,8 +320,11 @@ class Server:
         self._check(self._lib.pxd_serve_step(self._h, C.byref(f)))
         return f.value
 
-    def results(self):
-        """Pop every waiting result as (tag, ids, num_clusters, iterations)."""
+    def results(self, wait: bool = False):
+        """Pop the ready results as (tag, ids, num_clusters, iterations); with
+        wait, first wait until every finished graph's result is ready."""
+        if wait:
+            self._check(self._lib.pxd_serve_wait(self._h))
         out = []
         tag, n = C.c_uint64(), C.c_uint32()
         while self._check(self._lib.pxd_serve_next(self._h, C.byref(tag), C.byref(n))) == 0:

@@ -10,9 +10,12 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -186,6 +189,7 @@ struct pxd_engine {
     }
 
     void release() {
+        stop_ranker();
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
         lvl_sum.clear();
@@ -841,11 +845,69 @@ struct pxd_engine {
     uint64_t live_nodes = 0, live_arcs = 0;
     struct Done {
         uint64_t tag;
-        int32_t k, it;
-        std::vector<int32_t> ids;
+        int32_t k, it;                // k < 0: the graph failed the device check
+        std::vector<int32_t> lab;     // raw labels, then cluster ids once ranked
+        bool ranked;
+        uint32_t n;                   // node count (lab is moved out while it is ranked)
     };
+    // Finished results in completion order.  A worker thread ranks the labels
+    // (cuda/devPartition.h:798-823) off the stepping thread; next/take wait for
+    // the oldest one only.
     std::deque<Done> done;
-    uint32_t* d_stage = nullptr;      // ranges of the graph being admitted
+    std::mutex done_mu;
+    std::condition_variable done_cv;
+    std::thread ranker;
+    bool ranker_stop = false;
+
+    void push_done(Done&& D) {
+        {
+            std::lock_guard<std::mutex> lk(done_mu);
+            done.push_back(std::move(D));
+        }
+        done_cv.notify_all();
+    }
+    void rank_loop() {
+        std::unique_lock<std::mutex> lk(done_mu);
+        for (;;) {
+            Done* D = nullptr;
+            for (Done& d : done)
+                if (!d.ranked) { D = &d; break; }
+            if (!D) {
+                if (ranker_stop) return;
+                done_cv.wait(lk);
+                continue;
+            }
+            std::vector<int32_t> lab = std::move(D->lab);   // deque elements keep their address
+            lk.unlock();
+            std::vector<int32_t> ids(lab.size());
+            const int32_t k = dense_rank(lab.data(), (uint32_t)lab.size(), ids.data());
+            lk.lock();
+            D->lab = std::move(ids);
+            if (D->k >= 0) D->k = k;
+            D->ranked = true;
+            done_cv.notify_all();
+        }
+    }
+    void stop_ranker() {
+        if (!ranker.joinable()) return;
+        {
+            std::lock_guard<std::mutex> lk(done_mu);
+            ranker_stop = true;
+        }
+        done_cv.notify_all();
+        ranker.join();
+        ranker_stop = false;
+    }
+    // admissions wait as `pending` until the next step checks and appends them
+    struct Pending {
+        uint64_t tag;
+        uint32_t base, n, abase, arcs, slot, stage;
+    };
+    std::vector<Pending> pending;
+    uint32_t stage_used = 0;          // bump pointer into d_stage (N + G + 1 entries)
+    uint32_t* d_stage = nullptr;      // raw ranges of pending graphs
+    uint32_t* d_pbad = nullptr;       // device check result per pending graph
+    std::vector<uint32_t> h_pbad;
     std::vector<int32_t> h_giter;
 
     static bool take_range(std::map<uint64_t, uint64_t>& fl, uint64_t len, uint64_t* start) {
@@ -891,7 +953,11 @@ struct pxd_engine {
         h_gm.assign(G, 0);
         P.keep_original = 0;          // no reset: the input CSR is not kept
         CHK(allocate());
-        CHK(alloc(&d_stage, (size_t)N + 1));
+        CHK(alloc(&d_stage, (size_t)N + G + 1));
+        CHK(alloc(&d_pbad, G));
+        h_pbad.assign(G, 0);
+        pending.clear();
+        stage_used = 0;
         // heavy-row buffers sized for the smallest threshold (4 lanes per node)
         heavy_cap = A / HEAVY_CHUNK + A / 128u + 1u;
         CHK(alloc(&d_hcnt, N));
@@ -914,7 +980,9 @@ struct pxd_engine {
         for (uint32_t g = G; g-- > 0;) free_slots.push_back(g);
         live_graphs = 0;
         live_nodes = live_arcs = 0;
+        stop_ranker();
         done.clear();
+        ranker = std::thread([this] { rank_loop(); });
         h_giter.assign(G, 0);
         cur = acur = pcur = 0;
         ctc = 0;
@@ -931,7 +999,9 @@ struct pxd_engine {
         return sync();
     }
 
-    // PXD_FULL when the graph does not fit the free capacity right now
+    // PXD_FULL when the graph does not fit the free capacity right now.  The
+    // uploads are queued; the graph is checked on the device and joins at the
+    // next step (the caller's buffers must stay valid until that step returns).
     int serve_admit(uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges, uint64_t tag,
                     int32_t* slot_out) {
         if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
@@ -941,7 +1011,7 @@ struct pxd_engine {
         if (ranges[0] != 0 || ranges[n] != arcs)
             return set_err(PXD_ERR_ARG, "ranges[n] must equal 2m and ranges[0] == 0");
         if (n == 0) {                 // nothing to decompose: an empty partition right away
-            done.push_back(Done{tag, 0, 1, {}});
+            push_done(Done{tag, 0, 1, {}, true, 0u});
             return PXD_OK;
         }
         if (n > N || arcs > A) return set_err(PXD_ERR_ARG, "graph larger than the serving capacity");
@@ -952,36 +1022,19 @@ struct pxd_engine {
             give_range(free_nodes, nb, n);
             return PXD_FULL;
         }
-        // upload and check on the device before anything else reads the graph
-        HIPC(hipMemcpyAsync(d_stage, ranges, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
-        if (arcs) HIPC(hipMemcpyAsync(d_nbr + ab, edges, arcs * 4, hipMemcpyHostToDevice, st));
-        HIPC(hipMemsetAsync(&d_cnt->admit_bad, 0, 4, st));
-        k_admit_check<<<grid_for(std::max<uint64_t>(n, arcs), BS), BS, 0, st>>>(n, d_stage, arcs, d_nbr + ab,
-                                                                              &d_cnt->admit_bad);
-        HIPC(hipGetLastError());
-        uint32_t bad = 0;
-        CHK(read_u32(&d_cnt->admit_bad, &bad));   // also: the caller's buffers are copied
-        if (bad) {
-            give_range(free_nodes, nb, n);
-            give_range(free_arcs, ab, arcs);
-            return set_err(PXD_ERR_ARG, (bad & 1u) ? "ranges decrease" : "an arc points to a node >= n");
-        }
         const uint32_t g = free_slots.back();
         free_slots.pop_back();
         slots[g] = Slot{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, true};
-        k_admit_nodes<<<grid_for(n, BS), BS, 0, st>>>((uint32_t)nb, n, (uint32_t)ab, g, nc, d_stage, P.seed, d_row,
-                                                      d_deg, d_dint, d_x[cur], d_y[cur], d_xkey, d_label, d_alive,
-                                                      d_stamp, d_ngraph, d_pnix[0] + na, d_pclu[0] + na,
-                                                      d_alist[acur] + nalist);
-        if (arcs) k_rebase<<<grid_for(arcs, BS), BS, 0, st>>>(arcs, d_nbr + ab, (uint32_t)nb);
-        k_admit_cluster<<<1, 64, 0, st>>>(nc, na, n, g, (uint32_t)nb, (uint32_t)arcs, iter1, ct[ctc].cols(), d_gsmall,
-                                          d_gsmall_next, d_gsurv, d_giter, d_gseen, d_gstart);
+        // pending node counts stay within the free capacity, so the arena (N + G + 1) has room
+        const uint32_t k = (uint32_t)pending.size();
+        if (k == 0) HIPC(hipMemsetAsync(d_pbad, 0, G * 4, st));
+        HIPC(hipMemcpyAsync(d_stage + stage_used, ranges, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, st));
+        if (arcs) HIPC(hipMemcpyAsync(d_nbr + ab, edges, arcs * 4, hipMemcpyHostToDevice, st));
+        k_admit_check<<<grid_for(std::max<uint64_t>(n, arcs), BS), BS, 0, st>>>(n, d_stage + stage_used, arcs,
+                                                                              d_nbr + ab, d_pbad + k);
         HIPC(hipGetLastError());
-        na += n;
-        nalist += n;
-        nc += 1;
-        heavy_valid = false;
-        sweep_done = false;
+        pending.push_back(Pending{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, g, stage_used});
+        stage_used += n + 1;
         live_graphs++;
         live_nodes += n;
         live_arcs += arcs;
@@ -989,10 +1042,49 @@ struct pxd_engine {
         return PXD_OK;
     }
 
+    // checked pending graphs join (positions, active list, one cluster each);
+    // failed ones return their room and leave an error result
+    int join_pending() {
+        if (pending.empty()) return 0;
+        HIPC(hipMemcpyAsync(h_pbad.data(), d_pbad, pending.size() * 4, hipMemcpyDeviceToHost, st));
+        CHK(sync());                  // also: every queued upload has completed
+        for (size_t k = 0; k < pending.size(); ++k) {
+            const Pending& Q = pending[k];
+            if (h_pbad[k]) {
+                give_range(free_nodes, Q.base, Q.n);
+                give_range(free_arcs, Q.abase, Q.arcs);
+                slots[Q.slot].used = false;
+                free_slots.push_back(Q.slot);
+                live_graphs--;
+                live_nodes -= Q.n;
+                live_arcs -= Q.arcs;
+                push_done(Done{Q.tag, -1, 0, {}, true, 0u});
+                continue;
+            }
+            k_admit_nodes<<<grid_for(Q.n, BS), BS, 0, st>>>(Q.base, Q.n, Q.abase, Q.slot, nc, d_stage + Q.stage, P.seed,
+                                                            d_row, d_deg, d_dint, d_x[cur], d_y[cur], d_xkey, d_label,
+                                                            d_alive, d_stamp, d_ngraph, d_pnix[0] + na,
+                                                            d_pclu[0] + na, d_alist[acur] + nalist);
+            if (Q.arcs) k_rebase<<<grid_for(Q.arcs, BS), BS, 0, st>>>(Q.arcs, d_nbr + Q.abase, Q.base);
+            k_admit_cluster<<<1, 64, 0, st>>>(nc, na, Q.n, Q.slot, Q.base, Q.arcs, iter1, ct[ctc].cols(), d_gsmall,
+                                              d_gsmall_next, d_gsurv, d_giter, d_gseen, d_gstart);
+            HIPC(hipGetLastError());
+            na += Q.n;
+            nalist += Q.n;
+            nc += 1;
+        }
+        pending.clear();
+        stage_used = 0;
+        heavy_valid = false;
+        sweep_done = false;
+        return 0;
+    }
+
     // one iteration over every live graph; graphs that finished move to `done`
     int serve_step(uint32_t* finished) {
         if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
         uint32_t nf = 0;
+        CHK(join_pending());
         if (na > 0) {
             const double avg = live_nodes ? (double)live_arcs / (double)live_nodes : 0.0;
             gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
@@ -1005,15 +1097,12 @@ struct pxd_engine {
             CHK(cut());
             HIPC(hipMemcpyAsync(h_giter.data(), d_giter, G * 4, hipMemcpyDeviceToHost, st));
             CHK(sync());
-            std::vector<int32_t> lab;
             for (uint32_t g = 0; g < G; ++g) {
                 Slot& S = slots[g];
                 if (!S.used || h_giter[g] == 0) continue;
-                lab.resize(S.n);
-                HIPC(hipMemcpy(lab.data(), d_label + S.base, (size_t)S.n * 4, hipMemcpyDeviceToHost));
-                Done D{S.tag, 0, h_giter[g], std::vector<int32_t>(S.n)};
-                D.k = dense_rank(lab.data(), S.n, D.ids.data());
-                done.push_back(std::move(D));
+                Done D{S.tag, 0, h_giter[g], std::vector<int32_t>(S.n), false, S.n};
+                HIPC(hipMemcpy(D.lab.data(), d_label + S.base, (size_t)S.n * 4, hipMemcpyDeviceToHost));
+                push_done(std::move(D));
                 give_range(free_nodes, S.base, S.n);
                 give_range(free_arcs, S.abase, S.arcs);
                 S.used = false;
@@ -1386,19 +1475,33 @@ int pxd_serve_step(pxd_engine* e, uint32_t* finished) {
 
 int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n) {
     if (!e || !tag || !n) return set_err(PXD_ERR_ARG, "null argument");
-    if (e->done.empty()) return PXD_EMPTY;
+    std::lock_guard<std::mutex> lk(e->done_mu);
+    if (e->done.empty() || !e->done.front().ranked) return PXD_EMPTY;
     *tag = e->done.front().tag;
-    *n = (uint32_t)e->done.front().ids.size();
+    *n = e->done.front().n;
+    return PXD_OK;
+}
+
+int pxd_serve_wait(pxd_engine* e) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    std::unique_lock<std::mutex> lk(e->done_mu);
+    e->done_cv.wait(lk, [e] {
+        for (const pxd_engine::Done& d : e->done)
+            if (!d.ranked) return false;
+        return true;
+    });
     return PXD_OK;
 }
 
 int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations) {
     if (!e || !num_clusters || !iterations) return set_err(PXD_ERR_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(e->done_mu);
     if (e->done.empty()) return PXD_EMPTY;
+    e->done_cv.wait(lk, [e] { return e->done.front().ranked; });
     pxd_engine::Done& D = e->done.front();
-    if (!D.ids.empty()) {
+    if (D.n) {
         if (!ids_out) return set_err(PXD_ERR_ARG, "null output");
-        std::memcpy(ids_out, D.ids.data(), D.ids.size() * 4);
+        std::memcpy(ids_out, D.lab.data(), (size_t)D.n * 4);
     }
     *num_clusters = D.k;
     *iterations = D.it;
@@ -1413,7 +1516,10 @@ int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, 
     if (live_nodes) *live_nodes = e->live_nodes;
     if (live_arcs) *live_arcs = e->live_arcs;
     if (active_nodes) *active_nodes = e->na;
-    if (waiting_results) *waiting_results = (uint32_t)e->done.size();
+    if (waiting_results) {
+        std::lock_guard<std::mutex> lk(e->done_mu);
+        *waiting_results = (uint32_t)e->done.size();
+    }
     return PXD_OK;
 }
 

@@ -55,7 +55,7 @@ def drain(srv, pending, graphs, got, max_steps=100000):
         for tag, ids, k, it in srv.results():
             got[tag] = (ids, k, it)
         assert steps < max_steps
-    for tag, ids, k, it in srv.results():
+    for tag, ids, k, it in srv.results(wait=True):
         got[tag] = (ids, k, it)
     return refused
 
@@ -109,14 +109,15 @@ def test_serve_rejects_batch_calls_and_oversized_graphs():
     with pytest.raises(px.PxdError):
         srv.admit(px.sbm(600, 5, 0.08, 0.004, 9), 0)      # larger than the capacity
     assert srv.admit(px.from_edges(0, [], []), 5)         # empty graph: result at once
-    res = srv.results()
+    res = srv.results(wait=True)
     assert len(res) == 1 and res[0][0] == 5 and res[0][2] == 0 and res[0][3] == 1
     srv.close()
 
 
 def test_serve_refuses_invalid_graph_and_carries_on():
-    """Arcs outside the graph are caught on the device before the graph joins;
-    its room is returned and the server goes on serving."""
+    """Arcs outside the graph or decreasing ranges are caught on the device
+    before the graph joins: the result says so (num_clusters = -1), its room
+    is returned and the server goes on serving."""
     good = px.sbm(300, 3, 0.1, 0.01, 4)
     edges = good.edges.copy()
     edges[17] = good.n                                   # one arc past the last node
@@ -125,10 +126,14 @@ def test_serve_refuses_invalid_graph_and_carries_on():
     ranges[5], ranges[6] = ranges[6], ranges[5]          # decreasing ranges
     bad2 = px.Graph(good.n, good.m, ranges, good.edges.copy())
     srv = px.Server(node_capacity=400, arc_capacity=good.edges.size, graph_slots=1)
-    for g in (bad, bad2):
-        with pytest.raises(px.PxdError):
-            srv.admit(g, 1)
-    assert srv.stats()["live_graphs"] == 0
+    for tag, g in ((7, bad), (8, bad2)):
+        assert srv.admit(g, tag)
+        srv.step()
+        res = srv.results(wait=True)
+        assert [(r[0], r[2]) for r in res] == [(tag, -1)]
+        assert srv.stats()["live_graphs"] == 0
+    with pytest.raises(px.PxdError):                        # host-side check: ranges[n] != 2m
+        srv.admit(px.Graph(good.n, good.m + 1, good.ranges.copy(), good.edges.copy()), 9)
     got = {}
     drain(srv, [0], [good], got)
     srv.close()
