@@ -582,6 +582,15 @@ __global__ void __launch_bounds__(BS) k_tile_hist(uint32_t ntiles, const uint32_
     hist[L.hbase + threadIdx.x * L.ntiles + t] = h[threadIdx.x];
 }
 
+// One LSD pass over one tile: stable local ranking by the 8-bit digit at
+// `shift`, then the scatter to the digit's global offset.  Items are loaded
+// striped per wave (item i of lane l of wave w is tile position
+// w*WAVE*TILE_ITEMS + i*WAVE + l), so the wave's rounds visit the tile in
+// order: an item's rank is the number of same-digit lanes below it in its
+// round (eight ballots give the peer mask) plus the wave's running count of
+// that digit (kept in LDS, advanced by the lowest peer).  Per-digit prefixes
+// over the waves and the tile's digit starts give each item's local position;
+// items are staged there in LDS and written out in tile order.
 __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint32_t* __restrict__ tseg,
                                                      const uint32_t* __restrict__ tidx,
                                                      const LargeSeg* __restrict__ ls,
@@ -589,45 +598,73 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint
                                                      const uint32_t* __restrict__ ksrc,
                                                      const uint32_t* __restrict__ vsrc, int shift,
                                                      uint32_t* __restrict__ kdst, uint32_t* __restrict__ vdst) {
-    __shared__ uint64_t buf0[TILE], buf1[TILE];
-    __shared__ uint32_t h[256], hs[256], goff[256];
-    __shared__ uint32_t scratch[BS / WAVE];
-    __shared__ uint32_t s_or, s_and;
+    constexpr uint32_t NW = BS / WAVE;
+    __shared__ uint64_t buf[TILE];
+    __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t hs[256], goff[256];
+    __shared__ uint32_t scratch[NW];
     if (blockIdx.x >= ntiles) return;
     const LargeSeg L = ls[tseg[blockIdx.x]];
     const uint32_t t = tidx[blockIdx.x];
     const uint32_t b = L.start + t * TILE;
     const uint32_t cnt = min(L.start + L.len, b + (uint32_t)TILE) - b;
-    h[threadIdx.x] = 0u;
-    if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    for (uint32_t q = threadIdx.x; q < NW * 256u; q += BS) (&wcnt[0][0])[q] = 0u;
     __syncthreads();
-    uint32_t lor = 0u, land = 0xFFFFFFFFu;
-    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
-        const uint32_t k = ksrc[b + p];
-        buf0[p] = ((uint64_t)k << 32) | vsrc[b + p];
+    const uint64_t lt = (1ull << l) - 1ull;
+    uint64_t e[TILE_ITEMS];
+    uint32_t dg[TILE_ITEMS], rk[TILE_ITEMS];
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t k = ok ? ksrc[b + p] : 0u;
+        e[i] = ((uint64_t)k << 32) | (ok ? vsrc[b + p] : 0u);
         const uint32_t d = (k >> shift) & 255u;
-        atomicAdd(&h[d], 1u);
-        lor |= d;
-        land &= d;
+        uint64_t peer = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool one = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(one);
+            peer &= one ? bb : ~bb;
+        }
+        // the lowest peer reads and advances the wave's count; the others get it by shuffle
+        const bool lead = ok && (peer & lt) == 0ull;
+        const uint32_t leader = ok ? (uint32_t)__builtin_ctzll(peer) : l;
+        const uint32_t c0 = lead ? wcnt[w][d] : 0u;
+        const uint32_t c = __shfl(c0, (int)leader);
+        if (lead) wcnt[w][d] = c0 + (uint32_t)__popcll(peer);
+        dg[i] = d;
+        rk[i] = c + (uint32_t)__popcll(peer & lt);
     }
-    atomicOr(&s_or, lor);
-    atomicAnd(&s_and, land);
     __syncthreads();
     {
+        // thread t: digit t.  wcnt[w][t] becomes the count of digit t in waves < w
+        const uint32_t d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t ww = 0; ww < NW; ++ww) {
+            const uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = run;
+            run += c;
+        }
         uint32_t tot;
-        hs[threadIdx.x] = block_excl_scan<BS>(h[threadIdx.x], scratch, tot);
-        const uint32_t gi = L.hbase + threadIdx.x * L.ntiles + t;
-        goff[threadIdx.x] = hscan[gi] - hscan[L.hbase] + L.start;
+        hs[d] = block_excl_scan<BS>(run, scratch, tot);
+        goff[d] = hscan[L.hbase + d * L.ntiles + t] - hscan[L.hbase] + L.start;
     }
     __syncthreads();
-    const uint32_t bits = ((s_or ^ s_and) & 255u) << shift;
-    uint64_t* r = lds_split_sort(buf0, buf1, cnt, bits, scratch);
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+        if (p < cnt) buf[hs[dg[i]] + wcnt[w][dg[i]] + rk[i]] = e[i];
+    }
+    __syncthreads();
     for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
-        const uint64_t e = r[p];
-        const uint32_t d = (uint32_t)(e >> (32 + shift)) & 255u;
+        const uint64_t x = buf[p];
+        const uint32_t d = (uint32_t)(x >> (32 + shift)) & 255u;
         const uint32_t dst = goff[d] + (p - hs[d]);
-        kdst[dst] = (uint32_t)(e >> 32);
-        vdst[dst] = (uint32_t)e;
+        kdst[dst] = (uint32_t)(x >> 32);
+        vdst[dst] = (uint32_t)x;
     }
 }
 
