@@ -495,67 +495,115 @@ __global__ void __launch_bounds__(BS) k_sort_wave(uint32_t nlist, const uint32_t
     }
 }
 
-// Stable LDS split sort of cnt (<= TILE) packed (key << 32 | val) items on the
-// key bits set in `bits`; 256 threads, thread t owns items 8t..8t+7.
-// Returns the buffer holding the result.
-__device__ uint64_t* lds_split_sort(uint64_t* b0, uint64_t* b1, uint32_t cnt, uint32_t bits,
-                                    uint32_t* scratch) {
-    const uint32_t t = threadIdx.x;
-    for (int b = 0; b < 32; ++b) {
-        if (!((bits >> b) & 1u)) continue;
-        uint64_t e[TILE_ITEMS];
-        uint32_t z = 0;
+// Stable rank of a tile's items by the 8-bit digit at `shift`.  Item i of
+// lane l of wave w is tile position w*WAVE*TILE_ITEMS + i*WAVE + l (valid
+// below cnt), so a wave's rounds visit the tile in order: an item's rank is
+// the number of same-digit lanes below it in its round (eight ballots give the
+// peer mask) plus the wave's running count of that digit, which the lowest peer
+// reads and advances in LDS (the others get it by shuffle).  Per-digit prefixes
+// over the waves and the digit starts hs[] give pos[i], the item's position in
+// the digit-sorted tile.  Block-wide (every thread calls it); ends with a
+// barrier; hs stays valid until the next call.
+static_assert(BS == 256, "one thread per 8-bit digit");
+__device__ __forceinline__ void rank8(const uint64_t (&e)[TILE_ITEMS], uint32_t cnt, int shift,
+                                      uint32_t (*wcnt)[256], uint32_t* hs, uint32_t* scratch,
+                                      uint32_t (&pos)[TILE_ITEMS]) {
+    constexpr uint32_t NW = BS / WAVE;
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    for (uint32_t q = threadIdx.x; q < NW * 256u; q += BS) (&wcnt[0][0])[q] = 0u;
+    __syncthreads();
+    const uint64_t lt = (1ull << l) - 1ull;
+    uint32_t dg[TILE_ITEMS], rk[TILE_ITEMS];
 #pragma unroll
-        for (int i = 0; i < TILE_ITEMS; ++i) {
-            const uint32_t p = t * TILE_ITEMS + i;
-            e[i] = p < cnt ? b0[p] : 0ull;
-            if (p < cnt && !((e[i] >> (32 + b)) & 1ull)) ++z;
-        }
-        uint32_t Z;
-        uint32_t run = block_excl_scan<BS>(z, scratch, Z);
+    for (int i = 0; i < TILE_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t d = (uint32_t)(e[i] >> (32 + shift)) & 255u;
+        uint64_t peer = __ballot(ok);
 #pragma unroll
-        for (int i = 0; i < TILE_ITEMS; ++i) {
-            const uint32_t p = t * TILE_ITEMS + i;
-            if (p < cnt) {
-                const bool one = (e[i] >> (32 + b)) & 1ull;
-                const uint32_t dst = one ? (Z + p - run) : run;
-                if (!one) ++run;
-                b1[dst] = e[i];
-            }
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool one = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(one);
+            peer &= one ? bb : ~bb;
         }
-        __syncthreads();
-        uint64_t* tmp = b0; b0 = b1; b1 = tmp;
+        const bool lead = ok && (peer & lt) == 0ull;
+        const uint32_t leader = ok ? (uint32_t)__builtin_ctzll(peer) : l;
+        const uint32_t c0 = lead ? wcnt[w][d] : 0u;
+        const uint32_t c = __shfl(c0, (int)leader);
+        if (lead) wcnt[w][d] = c0 + (uint32_t)__popcll(peer);
+        dg[i] = d;
+        rk[i] = c + (uint32_t)__popcll(peer & lt);
     }
-    return b0;
+    __syncthreads();
+    {
+        const uint32_t d = threadIdx.x;   // wcnt[w][d] becomes the count of digit d in waves < w
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t ww = 0; ww < NW; ++ww) {
+            const uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = run;
+            run += c;
+        }
+        uint32_t tot;
+        hs[d] = block_excl_scan<BS>(run, scratch, tot);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; ++i) pos[i] = hs[dg[i]] + wcnt[w][dg[i]] + rk[i];
+    __syncthreads();
 }
 
 // one block per segment of 65..TILE items, sorted in place
 __global__ void __launch_bounds__(BS) k_sort_block(uint32_t nlist, const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ segs,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
-    __shared__ uint64_t buf0[TILE], buf1[TILE];
-    __shared__ uint32_t scratch[BS / WAVE];
+    constexpr uint32_t NW = BS / WAVE;
+    __shared__ uint64_t buf[TILE];
+    __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t hs[256];
+    __shared__ uint32_t scratch[NW];
     __shared__ uint32_t s_or, s_and;
     if (blockIdx.x >= nlist) return;
     const uint32_t c = list[blockIdx.x];
     const uint32_t s0 = segs[c], cnt = segs[c + 1] - s0;
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
     __syncthreads();
+    uint64_t e[TILE_ITEMS];
     uint32_t lor = 0u, land = 0xFFFFFFFFu;
-    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
-        const uint32_t k = key[s0 + p];
-        buf0[p] = ((uint64_t)k << 32) | val[s0 + p];
-        lor |= k;
-        land &= k;
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t k = ok ? key[s0 + p] : 0u;
+        e[i] = ((uint64_t)k << 32) | (ok ? val[s0 + p] : 0u);
+        if (ok) { lor |= k; land &= k; }
     }
     atomicOr(&s_or, lor);
     atomicAnd(&s_and, land);
     __syncthreads();
-    const uint32_t bits = s_or ^ s_and;
-    uint64_t* r = lds_split_sort(buf0, buf1, cnt, bits, scratch);
-    for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
-        key[s0 + p] = (uint32_t)(r[p] >> 32);
-        val[s0 + p] = (uint32_t)r[p];
+    const uint32_t bits = s_or ^ s_and;   // key bits that vary inside the segment
+    uint32_t pos[TILE_ITEMS];
+    for (int pass = 0; pass < 4; ++pass) {
+        if (((bits >> (8 * pass)) & 255u) == 0u) continue;     // a constant digit keeps the order
+        rank8(e, cnt, 8 * pass, wcnt, hs, scratch, pos);
+#pragma unroll
+        for (int i = 0; i < TILE_ITEMS; ++i)
+            if (w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l < cnt) buf[pos[i]] = e[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TILE_ITEMS; ++i) {
+            const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+            if (p < cnt) e[i] = buf[p];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TILE_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
+        if (p < cnt) {
+            key[s0 + p] = (uint32_t)(e[i] >> 32);
+            val[s0 + p] = (uint32_t)e[i];
+        }
     }
 }
 
@@ -582,15 +630,8 @@ __global__ void __launch_bounds__(BS) k_tile_hist(uint32_t ntiles, const uint32_
     hist[L.hbase + threadIdx.x * L.ntiles + t] = h[threadIdx.x];
 }
 
-// One LSD pass over one tile: stable local ranking by the 8-bit digit at
-// `shift`, then the scatter to the digit's global offset.  Items are loaded
-// striped per wave (item i of lane l of wave w is tile position
-// w*WAVE*TILE_ITEMS + i*WAVE + l), so the wave's rounds visit the tile in
-// order: an item's rank is the number of same-digit lanes below it in its
-// round (eight ballots give the peer mask) plus the wave's running count of
-// that digit (kept in LDS, advanced by the lowest peer).  Per-digit prefixes
-// over the waves and the tile's digit starts give each item's local position;
-// items are staged there in LDS and written out in tile order.
+// One LSD pass over one tile (rank8), then the scatter to the digit's global
+// offset: items are staged in LDS at their rank and written out in tile order.
 __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint32_t* __restrict__ tseg,
                                                      const uint32_t* __restrict__ tidx,
                                                      const LargeSeg* __restrict__ ls,
@@ -609,55 +650,19 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint
     const uint32_t b = L.start + t * TILE;
     const uint32_t cnt = min(L.start + L.len, b + (uint32_t)TILE) - b;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    for (uint32_t q = threadIdx.x; q < NW * 256u; q += BS) (&wcnt[0][0])[q] = 0u;
-    __syncthreads();
-    const uint64_t lt = (1ull << l) - 1ull;
+    goff[threadIdx.x] = hscan[L.hbase + threadIdx.x * L.ntiles + t] - hscan[L.hbase] + L.start;
     uint64_t e[TILE_ITEMS];
-    uint32_t dg[TILE_ITEMS], rk[TILE_ITEMS];
 #pragma unroll
     for (int i = 0; i < TILE_ITEMS; ++i) {
         const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
         const bool ok = p < cnt;
-        const uint32_t k = ok ? ksrc[b + p] : 0u;
-        e[i] = ((uint64_t)k << 32) | (ok ? vsrc[b + p] : 0u);
-        const uint32_t d = (k >> shift) & 255u;
-        uint64_t peer = __ballot(ok);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const bool one = (d >> bit) & 1u;
-            const uint64_t bb = __ballot(one);
-            peer &= one ? bb : ~bb;
-        }
-        // the lowest peer reads and advances the wave's count; the others get it by shuffle
-        const bool lead = ok && (peer & lt) == 0ull;
-        const uint32_t leader = ok ? (uint32_t)__builtin_ctzll(peer) : l;
-        const uint32_t c0 = lead ? wcnt[w][d] : 0u;
-        const uint32_t c = __shfl(c0, (int)leader);
-        if (lead) wcnt[w][d] = c0 + (uint32_t)__popcll(peer);
-        dg[i] = d;
-        rk[i] = c + (uint32_t)__popcll(peer & lt);
+        e[i] = ok ? (((uint64_t)ksrc[b + p] << 32) | vsrc[b + p]) : 0ull;
     }
-    __syncthreads();
-    {
-        // thread t: digit t.  wcnt[w][t] becomes the count of digit t in waves < w
-        const uint32_t d = threadIdx.x;
-        uint32_t run = 0;
+    uint32_t pos[TILE_ITEMS];
+    rank8(e, cnt, shift, wcnt, hs, scratch, pos);
 #pragma unroll
-        for (uint32_t ww = 0; ww < NW; ++ww) {
-            const uint32_t c = wcnt[ww][d];
-            wcnt[ww][d] = run;
-            run += c;
-        }
-        uint32_t tot;
-        hs[d] = block_excl_scan<BS>(run, scratch, tot);
-        goff[d] = hscan[L.hbase + d * L.ntiles + t] - hscan[L.hbase] + L.start;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < TILE_ITEMS; ++i) {
-        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
-        if (p < cnt) buf[hs[dg[i]] + wcnt[w][dg[i]] + rk[i]] = e[i];
-    }
+    for (int i = 0; i < TILE_ITEMS; ++i)
+        if (w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l < cnt) buf[pos[i]] = e[i];
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
         const uint64_t x = buf[p];
