@@ -583,7 +583,7 @@ int ox_final_partition(ox_state* s, int32_t* ids_out) {
     int32_t* u = (int32_t*)malloc((size_t)(n ? n : 1) * 4);
     memcpy(u, s->labels, (size_t)n * 4);
     /* sort ascending (insertion-free: use merge sort through idx on u32 keys) */
-    uint64_t* k = (uint64_t*)malloc((size_t)(n ? n : 1) * 8);
+    uint64_t* k = (uint64_t*)calloc((size_t)(n ? n : 1), 8);
     for (uint32_t i = 0; i < n; ++i) { k[i] = (uint64_t)((uint32_t)u[i] ^ 0x80000000u); s->idx[i] = i; }
     msort_u64(s->idx, s->tmp, 0, n, k);
     int32_t* uniq = (int32_t*)malloc((size_t)(n ? n : 1) * 4);

@@ -409,7 +409,8 @@ int pxg_read(const char* path, pxg_graph* out) {
     out->n = hdr[0];
     out->m = hdr[1];
     out->ranges = (uint32_t*)malloc(((size_t)hdr[0] + 1) * 4);
-    out->edges = (uint32_t*)malloc((2ull * hdr[1] ? 2ull * hdr[1] : 1) * 4);
+    const uint64_t narcs = 2ull * hdr[1];
+    out->edges = (uint32_t*)malloc((narcs ? narcs : 1) * 4);
     int ok = out->ranges && out->edges &&
              fread(out->ranges, 4, (size_t)hdr[0] + 1, f) == (size_t)hdr[0] + 1 &&
              fread(out->edges, 4, 2ull * hdr[1], f) == 2ull * hdr[1];
