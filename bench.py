@@ -308,13 +308,19 @@ def main():
     dm.expanderDecomposition()
     launches, walk_ms, walk_bytes = dm.walkStats()
     dm.enableWalkTiming(False)
+    tuning, ctuning, tunings = dm.walkTuning()
     achieved = walk_bytes / (walk_ms / 1e3) / 1e9 if walk_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": profiled_traffic(a.workload) if a.graphs is None else None,
                 "kernel": "k_walk", "launches": launches,
                 "avg_launch_us": round(walk_ms * 1e3 / max(launches, 1), 2),
-                "alg_bytes_per_launch": round(walk_bytes / max(launches, 1))}
+                "alg_bytes_per_launch": round(walk_bytes / max(launches, 1)),
+                # walk kernel chosen per log2(active-list length) class by the
+                # online tuner (first decomposition run): 0..5 = 2..64 lanes
+                # per row, 6/7/8 = LDS window with 1/2/4 lanes per row
+                "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
+                "tunings": tunings}
     dm.close()
 
     # ---------------------------------------------------------------- plain
@@ -327,6 +333,7 @@ def main():
         for i, g in enumerate(graphs):
             d1 = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
             d1.initialize(g)
+            d1.setWalkVariant(-2)   # the degree rule: no tuning launches inside the plain timing
             t = time.perf_counter()
             try:
                 it1 = d1.expanderDecomposition()

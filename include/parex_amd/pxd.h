@@ -146,6 +146,21 @@ int pxd_last_run_ms(pxd_engine* e, double* ms);
 int pxd_enable_walk_timing(pxd_engine* e, int enable);
 int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes);
 
+/* Kernel choice for the walk and the cut contribution, the two per-arc
+ * passes of an iteration (no counterpart in the reference, whose kernels have
+ * one launch shape each, cuda/devRandomWalk.h:369-392, cuda/devSweepCut.h:
+ * 486-499).  variant 0..5: one row per 2, 4, 8, 16, 32, 64 lanes; 6, 7, 8:
+ * the LDS-window kernels with 1, 2, 4 lanes per row; -1 (default): online
+ * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
+ * (b >= 16) every candidate is launched once on the current state and timed,
+ * the fastest is kept for that pass and length class (the tables survive
+ * pxd_reset); -2: the degree rule without tuning.  Every variant computes the
+ * same bits.  pxd_walk_tuning copies the tables (entries 0..32: walk per
+ * class b, 33..65: cut contribution; -1 = not tuned) and the number of
+ * tunings run. */
+int pxd_set_walk_variant(pxd_engine* e, int variant);
+int pxd_walk_tuning(pxd_engine* e, int32_t* table, uint32_t capacity, uint64_t* tunings);
+
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference (cuda/interface.h runs one graph per
  * manager, main.cpp:44-50 one file per process).  Graphs join a running engine

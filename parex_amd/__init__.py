@@ -130,6 +130,8 @@ def pxd() -> C.CDLL:
         lib.pxd_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
         lib.pxd_enable_walk_timing.argtypes = [P, C.c_int]
         lib.pxd_walk_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.pxd_set_walk_variant.argtypes = [P, C.c_int]
+        lib.pxd_walk_tuning.argtypes = [P, _i32p, C.c_uint32, C.POINTER(C.c_uint64)]
         lib.pxd_serve_create.argtypes = [C.POINTER(_Params), C.c_uint32, C.c_uint64, C.c_uint32,
                                          C.POINTER(C.c_void_p)]
         lib.pxd_serve_admit.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint64,
@@ -153,6 +155,7 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
+    "pxd_set_walk_variant", "pxd_walk_tuning",
     "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
     "pxd_serve_stats",
 ]
@@ -515,6 +518,23 @@ class DeviceManager:
         a, b, c = C.c_uint64(), C.c_double(), C.c_double()
         self._check(self._lib.pxd_walk_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return int(a.value), float(b.value), float(c.value)
+
+    def setWalkVariant(self, variant: int):
+        """Pin the kernels of the walk and the cut contribution (0..5: 2..64
+        lanes per row; 6, 7, 8: LDS window with 1, 2, 4 lanes per row), -1 =
+        online tuning (default), -2 = the degree rule.  Every variant computes
+        the same bits."""
+        self._check(self._lib.pxd_set_walk_variant(self._h, int(variant)))
+
+    def walkTuning(self):
+        """({log2 list-length class: walk variant}, {class: cut-contribution
+        variant}, number of tunings run)."""
+        t = np.full(66, -1, np.int32)
+        n = C.c_uint64()
+        self._check(self._lib.pxd_walk_tuning(self._h, t, 66, C.byref(n)))
+        walk = {b: int(v) for b, v in enumerate(t[:33]) if v >= 0}
+        contrib = {b: int(v) for b, v in enumerate(t[33:]) if v >= 0}
+        return walk, contrib, int(n.value)
 
     # snake_case aliases
     iterate_random_walk = iterateRandomWalk

@@ -181,10 +181,31 @@ struct pxd_engine {
     double walk_ms = 0.0, walk_bytes = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+    // kernel choice for the two arc passes of an iteration (0 = walk, 1 = cut
+    // contribution): variants 0..5 = one row per 2, 4, ..., 64 lanes
+    // (k_walk / k_contrib), 6..8 = the LDS-window kernels with 1, 2, 4 lanes
+    // per row.  walk_force >= 0 pins one variant for both passes; -1 = online
+    // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
+    // every candidate is launched once on the current state (both passes are
+    // pure functions of it, all variants give the same bits) and timed, and
+    // the fastest is kept for that pass and length class; -2 = the degree rule
+    // without tuning.  The tables survive pxd_reset, so repeated
+    // decompositions of the same input reuse them.
+    static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_COUNT = 9;
+    static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
+    int walk_force = -1;
+    int tuned[2][33];
+    uint64_t walk_tunings = 0;
+    double avg_deg = 0.0;
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;
+
+    pxd_engine() { std::fill(&tuned[0][0], &tuned[0][0] + 66, -1); }
     ~pxd_engine() {
         release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (tev0) (void)hipEventDestroy(tev0);
+        if (tev1) (void)hipEventDestroy(tev1);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -336,6 +357,7 @@ struct pxd_engine {
         release();
         serving = false;
         HIPC(hipSetDevice(dev));
+        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);   // a new input: tune again
         G = g;
         h_gnode.assign(G + 1, 0);
         h_garc.assign(G + 1, 0);
@@ -447,6 +469,7 @@ struct pxd_engine {
         iter1 = 0;
         sweep_done = false;
         const double avg = N ? (double)A / (double)N : 0.0;
+        avg_deg = avg;
         // lanes per node: about four arcs per lane (the arc loops keep four in flight)
         gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
         // rows longer than 32 arcs per lane leave the lane-group kernels
@@ -559,10 +582,97 @@ struct pxd_engine {
                                                                          d_y[cur], d_x[cur], d_x[cur ^ 1],
                                                                          d_y[cur ^ 1], d_xkey, heavy_thr, d_cnt);
     }
+    template <int GSZ, int U, int WIN, int CHUNK>
+    void launch_walk_win() {
+        const uint32_t nch = cdiv(nalist, CHUNK);
+        const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
+        k_walk_win<GSZ, U, WIN, CHUNK><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur],
+                                                            d_x[cur], d_x[cur ^ 1], d_y[cur ^ 1], d_xkey, heavy_thr,
+                                                            N, d_cnt);
+    }
+    void launch_walk_variant(int v) {
+        switch (v) {
+            case 0: launch_walk<2>(); break;
+            case 1: launch_walk<4>(); break;
+            case 2: launch_walk<8>(); break;
+            case 3: launch_walk<16>(); break;
+            case 4: launch_walk<32>(); break;
+            case 5: launch_walk<64>(); break;
+            case WV_WIN1: launch_walk_win<1, 8, 20480, 4096>(); break;
+            case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
+            default: launch_walk_win<4, 8, 16384, 4096>(); break;
+        }
+    }
+    static int lanes_variant(int lanes) {
+        int v = 0;
+        while ((2 << v) < lanes && v < 5) ++v;
+        return v;
+    }
+    // degree rule (measured on one MI355X): very sparse rows take 2 lanes
+    // each, up to 24 arcs per row the LDS window with 2 lanes, longer rows
+    // about four arcs per lane
+    int default_variant() const {
+        if (avg_deg <= 6.0) return WV_LANES2;
+        if (avg_deg <= 24.0) return WV_WIN2;
+        return lanes_variant(gs);
+    }
+    // variant for `pass` at the current list length; launch(v) runs candidate v
+    template <class F>
+    int pick_variant(int pass, F&& launch) {
+        if (walk_force >= 0) return walk_force;
+        const int def = default_variant();
+        if (walk_force == -2 || nalist < TUNE_MIN) return def;
+        const int b = 31 - __builtin_clz(nalist);
+        if (tuned[pass][b] >= 0) return tuned[pass][b];
+        std::vector<int> cand;
+        const int lv = lanes_variant(gs);
+        for (int v = std::max(0, lv - 1); v <= std::min(5, lv + 1); ++v) cand.push_back(v);
+        if (gs <= 8) cand.push_back(WV_WIN2);
+        if (gs <= 4) cand.push_back(WV_WIN1);
+        if (gs >= 4 && gs <= 8) cand.push_back(WV_WIN4);
+        int best = def;
+        float best_ms = 0.f;
+        for (int v : cand) {
+            if (hipEventRecord(tev0, st) != hipSuccess) return def;
+            launch(v);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(tev1, st) != hipSuccess ||
+                hipEventSynchronize(tev1) != hipSuccess)
+                return def;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, tev0, tev1) != hipSuccess) return def;
+            if (best_ms == 0.f || ms < best_ms) {
+                best = v;
+                best_ms = ms;
+            }
+        }
+        tuned[pass][b] = best;
+        ++walk_tunings;
+        return best;
+    }
     template <int GSZ>
     void launch_contrib() {
         k_contrib<GSZ><<<grid_xcd(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
                                                                      d_xkey, d_ediff, heavy_thr);
+    }
+    template <int GSZ, int U, int WIN, int CHUNK>
+    void launch_contrib_win() {
+        const uint32_t nch = cdiv(nalist, CHUNK);
+        const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
+        k_contrib_win<GSZ, U, WIN, CHUNK><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_xkey,
+                                                               d_ediff, heavy_thr, N);
+    }
+    void launch_contrib_variant(int v) {
+        switch (v) {
+            case 0: launch_contrib<2>(); break;
+            case 1: launch_contrib<4>(); break;
+            case 2: launch_contrib<8>(); break;
+            case 3: launch_contrib<16>(); break;
+            case 4: launch_contrib<32>(); break;
+            case 5: launch_contrib<64>(); break;
+            case WV_WIN1: launch_contrib_win<1, 8, 40960, 4096>(); break;
+            case WV_WIN2: launch_contrib_win<2, 8, 40960, 4096>(); break;
+            default: launch_contrib_win<4, 8, 32768, 4096>(); break;
+        }
     }
     template <int GSZ>
     void launch_link(const uint32_t* list, uint32_t nl) {
@@ -600,9 +710,10 @@ struct pxd_engine {
     int walk() {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (nalist) {
+            const int wv = pick_variant(0, [this](int v) { launch_walk_variant(v); });
             if (walk_timing) HIPC(hipEventRecord(ev0, st));
             CHK(heavy_map());
-            PXD_DISPATCH_GS(launch_walk);
+            launch_walk_variant(wv);
             HIPC(hipGetLastError());
             if (heavy_thr != 0xFFFFFFFFu) {
                 const uint32_t* H = d_hoff + nalist;   // total chunk count, on the device
@@ -649,7 +760,7 @@ struct pxd_engine {
         HIPC(hipGetLastError());
         CHK(seg_sort(T.cs, nc));
         CHK(heavy_map());
-        PXD_DISPATCH_GS(launch_contrib);
+        launch_contrib_variant(pick_variant(1, [this](int v) { launch_contrib_variant(v); }));
         HIPC(hipGetLastError());
         if (heavy_thr != 0xFFFFFFFFu) {
             k_contrib_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nalist, d_hrow, d_hoff, d_alist[acur], d_row, d_dint,
@@ -994,6 +1105,7 @@ struct pxd_engine {
         gs = 4;
         heavy_thr = 128u;
         heavy_valid = false;
+        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
         serving = true;
         inited = true;
         return sync();
@@ -1087,6 +1199,7 @@ struct pxd_engine {
         CHK(join_pending());
         if (na > 0) {
             const double avg = live_nodes ? (double)live_arcs / (double)live_nodes : 0.0;
+            avg_deg = avg;
             gs = avg <= 24.0 ? 4 : avg <= 48.0 ? 8 : avg <= 96.0 ? 16 : avg <= 192.0 ? 32 : 64;
             if (heavy_thr != 32u * gs) {
                 heavy_thr = 32u * gs;
@@ -1200,7 +1313,8 @@ int pxd_create(const pxd_params* params, pxd_engine** out) {
                        hipGetErrorString(err));
     }
     if (hipSetDevice(e->dev) != hipSuccess || hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipEventCreate(&e->tev0) != hipSuccess || hipEventCreate(&e->tev1) != hipSuccess) {
         delete e;
         return set_err(PXD_ERR_HIP, "stream/event creation failed");
     }
@@ -1450,6 +1564,20 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
     if (launches) *launches = e->walk_launches;
     if (total_ms) *total_ms = e->walk_ms;
     if (alg_bytes) *alg_bytes = e->walk_bytes;
+    return 0;
+}
+
+int pxd_set_walk_variant(pxd_engine* e, int variant) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    if (variant < -2 || variant >= pxd_engine::WV_COUNT) return set_err(PXD_ERR_ARG, "walk variant %d out of range", variant);
+    e->walk_force = variant;
+    return 0;
+}
+
+int pxd_walk_tuning(pxd_engine* e, int32_t* table, uint32_t capacity, uint64_t* tunings) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    for (uint32_t b = 0; b < capacity && b < 66; ++b) table[b] = e->tuned[b / 33][b % 33];
+    if (tunings) *tunings = e->walk_tunings;
     return 0;
 }
 

@@ -19,6 +19,7 @@
 namespace pxd {
 
 constexpr int BS = 256;                 // default block size (4 waves)
+constexpr int WBS = 1024;               // LDS-window kernels: one 16-wave workgroup per CU
 constexpr int TILE = 2048;              // LDS sort tile: 256 threads x 8 items
 constexpr int TILE_ITEMS = 8;
 constexpr int SCAN_ITEMS = 4;           // scans: 1024 elements per block
@@ -273,6 +274,95 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
     }
 }
 
+// Cut contribution with an LDS window of xkey: the layout of k_walk_win
+// (chunks of CH list entries per workgroup, WIN keys around a chunk staged
+// with coalesced 16-byte loads when its nodes span less than WIN/2, 64 rows
+// per wave in GS rounds, each row's count returned to its owner lane).
+template <int GS, int U, int WIN, int CH>
+__global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t* __restrict__ alist,
+                                                     const uint32_t* __restrict__ row,
+                                                     const uint32_t* __restrict__ dint,
+                                                     const uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ xkey,
+                                                     int32_t* __restrict__ ediff, uint32_t heavy, uint32_t N) {
+    static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of key quads, lane groups within a wave");
+    __shared__ uint32_t sk[WIN];
+    constexpr uint32_t NPR = 64 / GS;
+    constexpr uint32_t NW = WBS / 64;
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    const uint32_t g = lane / GS, gl = lane % GS;
+    const uint32_t nch = (na + CH - 1) / CH;
+    const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
+    const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
+    const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
+    for (uint32_t c = cb + lb; c < ce; c += nb) {
+        const uint32_t c0 = c * CH, c1 = min(c0 + CH, na);
+        const uint32_t uf = alist[c0], ul = alist[c1 - 1];
+        const bool use = ul - uf < (uint32_t)WIN / 2;
+        uint32_t wb = 0;
+        if (use) {
+            const uint32_t mid = uf / 2 + ul / 2;
+            wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
+            if (N <= (uint32_t)WIN) wb = 0;
+            else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
+            wb &= ~3u;
+            for (uint32_t k = 4 * threadIdx.x; k < (uint32_t)WIN; k += 4 * WBS) {
+                if (wb + k + 3 < N) {
+                    const uint4 t = *(const uint4*)(xkey + wb + k);
+                    sk[k] = t.x;
+                    sk[k + 1] = t.y;
+                    sk[k + 2] = t.z;
+                    sk[k + 3] = t.w;
+                } else {
+                    for (uint32_t q = 0; q < 4; ++q)
+                        if (wb + k + q < N) sk[k + q] = xkey[wb + k + q];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t ib = c0 + w * 64; ib < c1; ib += NW * 64) {
+            const uint32_t i = ib + lane;
+            const bool ok = i < c1;
+            const uint32_t u = ok ? alist[i] : alist[c0];
+            const uint32_t st = row[u];
+            const uint32_t dd = dint[u];
+            const bool light = dd <= heavy;
+            const uint32_t d = ok && light ? dd : 0u;
+            const uint32_t mk = xkey[u];
+            int32_t res = 0;
+#pragma unroll
+            for (int r = 0; r < GS; ++r) {
+                const uint32_t s = r * NPR + g;
+                const uint32_t rs = __shfl(st, s), rd = __shfl(d, s), ru = __shfl(u, s), rk = __shfl(mk, s);
+                int32_t cnt = 0;
+                for (uint32_t j = gl; j < rd; j += U * GS) {
+                    uint32_t v[U], k[U];
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t jj = j + q * GS;
+                        v[q] = jj < rd ? nbr[rs + jj] : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t off = v[q] - wb;
+                        k[q] = j + q * GS < rd ? ((use && off < (uint32_t)WIN) ? sk[off] : xkey[v[q]]) : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const bool before = (k[q] != rk) ? (k[q] < rk) : (v[q] < ru);
+                        if (j + q * GS < rd) cnt += before ? -1 : 1;
+                    }
+                }
+                cnt = group_sum<GS>(cnt);
+                const int32_t got = __shfl(cnt, (lane % NPR) * GS);
+                if (lane / NPR == (uint32_t)r) res = got;
+            }
+            if (ok) ediff[u] = light ? res : 0;   // k_contrib_heavy adds the long rows' chunks
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ heavy rows
 
 // Rows above the light kernels' threshold are split into chunks of
@@ -295,6 +385,112 @@ __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, co
         const uint32_t c = hcnt[i], o = hoff[i];
         for (uint32_t k = 0; k < c; ++k) hrow[o + k] = i;
     }
+}
+
+// Walk with an LDS window of y.  A workgroup of WBS threads takes chunks of CH
+// consecutive node-list entries.  When a chunk's nodes span less than WIN/2
+// indices, the WIN values of y around them are staged into LDS with coalesced
+// 16-byte loads, and every neighbour inside the window is read from LDS;
+// the rest (and every neighbour of a wider chunk) from global memory.  Graphs
+// whose neighbours sit near in index order (planted blocks, grids) then turn
+// most per-arc 128-byte line fetches into LDS reads.  Each wave takes 64
+// entries at a time: lane l loads the metadata of entry l (coalesced), the 64
+// rows are summed in GS rounds of 64/GS rows with GS lanes per row and U
+// gathers in flight per lane, each exact row sum goes back to its owner lane,
+// and all 64 lanes round and store together.  Same arithmetic as k_walk, so
+// the same bits.  Rows above `heavy` are left to k_walk_heavy.
+template <int GS, int U, int WIN, int CH>
+__global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* __restrict__ alist,
+                                                  const uint32_t* __restrict__ row,
+                                                  const uint32_t* __restrict__ dint,
+                                                  const uint32_t* __restrict__ nbr,
+                                                  const double* __restrict__ y,
+                                                  const double* __restrict__ xc, double* __restrict__ xn,
+                                                  double* __restrict__ yn, uint32_t* __restrict__ xkey,
+                                                  uint32_t heavy, uint32_t N, Counters* cnt) {
+    static_assert(WIN % 2 == 0 && 64 % GS == 0, "window of double pairs, lane groups within a wave");
+    __shared__ double sy[WIN];
+    constexpr uint32_t NPR = 64 / GS;           // rows per round
+    constexpr uint32_t NW = WBS / 64;
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    const uint32_t g = lane / GS, gl = lane % GS;
+    // XCD-aware chunk split (see xcd_span): XCD x takes the x-th eighth of the chunks
+    const uint32_t nch = (na + CH - 1) / CH;
+    const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
+    const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
+    const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
+    uint32_t ovf = 0;
+    for (uint32_t c = cb + lb; c < ce; c += nb) {
+        const uint32_t c0 = c * CH, c1 = min(c0 + CH, na);
+        const uint32_t uf = alist[c0], ul = alist[c1 - 1];   // the list is ascending
+        const bool use = ul - uf < (uint32_t)WIN / 2;
+        uint32_t wb = 0;
+        if (use) {
+            const uint32_t mid = uf / 2 + ul / 2;
+            wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
+            if (N <= (uint32_t)WIN) wb = 0;
+            else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
+            wb &= ~1u;
+            for (uint32_t k = 2 * threadIdx.x; k < (uint32_t)WIN; k += 2 * WBS) {
+                if (wb + k + 1 < N) {
+                    const double2 t = *(const double2*)(y + wb + k);
+                    sy[k] = t.x;
+                    sy[k + 1] = t.y;
+                } else if (wb + k < N) {
+                    sy[k] = y[wb + k];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t ib = c0 + w * 64; ib < c1; ib += NW * 64) {
+            const uint32_t i = ib + lane;
+            const bool ok = i < c1;
+            const uint32_t u = ok ? alist[i] : alist[c0];
+            const uint32_t st = row[u];
+            const uint32_t dd = dint[u];
+            const bool mine = ok && dd <= heavy;
+            const uint32_t d = mine ? dd : 0u;
+            const double xo = xc[u];
+            Fix128 res = {0ull, 0ull};
+#pragma unroll
+            for (int r = 0; r < GS; ++r) {
+                const uint32_t s = r * NPR + g;
+                const uint32_t rs = __shfl(st, s), rd = __shfl(d, s);
+                Fix128 acc = {0ull, 0ull};
+                for (uint32_t j = gl; j < rd; j += U * GS) {
+                    uint32_t q[U];
+                    double yv[U];
+#pragma unroll
+                    for (int k = 0; k < U; ++k) {
+                        const uint32_t jj = j + k * GS;
+                        q[k] = jj < rd ? nbr[rs + jj] : 0u;
+                    }
+#pragma unroll
+                    for (int k = 0; k < U; ++k) {
+                        const uint32_t jj = j + k * GS;
+                        const uint32_t off = q[k] - wb;
+                        yv[k] = jj < rd ? ((use && off < (uint32_t)WIN) ? sy[off] : y[q[k]]) : 0.0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < U; ++k) fix_add_term(acc, yv[k], ovf);
+                }
+                acc = group_sum_fix<GS>(acc);
+                const uint32_t src = (lane % NPR) * GS;
+                const uint64_t lo = __shfl(acc.lo, src), hi = __shfl(acc.hi, src);
+                if (lane / NPR == (uint32_t)r) res = Fix128{lo, hi};
+            }
+            if (mine) {
+                if (d > (1u << 24)) ovf = 1;
+                const double sum = fix_to_f64(res);
+                const double v = sum * (1.0 - RW_STAY) + xo * RW_STAY;
+                xn[u] = v;
+                yn[u] = d ? v / (double)d : 0.0;
+                xkey[u] = ord_f32((float)v);
+            }
+        }
+        __syncthreads();              // the window is restaged for the next chunk
+    }
+    if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
 }
 
 // sum of a Fix128 over the workgroup, valid in thread 0

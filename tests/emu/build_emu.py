@@ -34,7 +34,7 @@ def _split_top(s):
 
 def transform(src: str) -> str:
     out, i = [], 0
-    pat = re.compile(r"([A-Za-z_][A-Za-z_0-9]*(?:<[A-Za-z_0-9]+>)?)<<<")
+    pat = re.compile(r"([A-Za-z_][A-Za-z_0-9]*(?:<[A-Za-z_0-9, ]+>)?)<<<")
     while True:
         m = pat.search(src, i)
         if not m:

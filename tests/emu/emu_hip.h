@@ -28,6 +28,8 @@
 #define __shared__ static
 
 struct emu_dim3 { uint32_t x = 0, y = 1, z = 1; };
+struct double2 { double x, y; };
+struct uint4 { uint32_t x, y, z, w; };
 extern thread_local emu_dim3 threadIdx, blockIdx, blockDim, gridDim;
 
 namespace emu {

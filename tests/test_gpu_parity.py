@@ -76,6 +76,57 @@ def test_walk_lockstep(name):
         assert bad.size == 0, f"{name} step {step}: {bad.size} values differ, first {bad[:5]} {a[bad[:3]]} {b[bad[:3]]}"
 
 
+WALK_VARIANT_GRAPHS = {
+    "sbm_c1": lambda: px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582),
+    "hub3000": lambda: hub(3000),
+    # neighbours spread over the whole index range: gathers outside every window
+    "rmat14": lambda: px.rmat(14, 16, 0.57, 0.19, 0.19, 7),
+    # 30,000 nodes in blocks of 3,000: windows that cover a block, chunks that span two
+    "sbm30k": lambda: px.sbm(30000, 10, 8 / 2999, 1 / 27000, 3),
+}
+
+
+@pytest.mark.parametrize("variant", range(9))
+def test_walk_variants_lockstep(variant):
+    """Every kernel variant of the walk and the cut contribution -- 2..64
+    lanes per row, the LDS window with 1, 2, 4 lanes per row -- gives the
+    oracle's bits (walk values, NodeData prefixes, sweep cuts, labels),
+    through cuts and arc disabling."""
+    for name, make in WALK_VARIANT_GRAPHS.items():
+        g = make()
+        dm = _engine(g)
+        dm.setWalkVariant(variant)
+        ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+        for it in range(1, 9):
+            if ox.counts()[1] == 0:
+                break
+            dm.iterateRandomWalk(); ox.walk()
+            a, b = dm.readRandomWalkValuesF64(), ox.values()
+            bad = np.nonzero(_bits(a) != _bits(b))[0]
+            assert bad.size == 0, f"{name} variant {variant} iter {it}: {bad.size} values differ, first {bad[:5]}"
+            dm.computeSweepCuts(); ox.sweep()
+            _compare_sweep(dm, ox, f"{name} variant {variant} iter {it} sweep")
+            dm.cutClusters(); ox.cut()
+            assert np.array_equal(dm.downloadLabels(), ox.labels()), f"{name} variant {variant} iter {it}: labels"
+        dm.close()
+
+
+def test_walk_tuning_runs_and_keeps_results():
+    """Online tuning on a list of 2^17 nodes: a table entry is chosen, and the
+    decomposition equals the one under the degree rule."""
+    g = px.sbm(150000, 30, 14 / 4999, 1 / 145000, 17)
+    dm = _engine(g, record_node_data=False)
+    it = dm.expanderDecomposition()
+    table, ctable, tunings = dm.walkTuning()
+    assert tunings >= 2 and 17 in table and 0 <= table[17] <= 8 and 17 in ctable and 0 <= ctable[17] <= 8
+    ids, k = dm.getFinalPartition()
+    dm.reset()
+    dm.setWalkVariant(-2)
+    assert dm.expanderDecomposition() == it
+    ids2, k2 = dm.getFinalPartition()
+    assert k2 == k and np.array_equal(ids2, ids)
+
+
 def _compare_sweep(dm, ox, tag):
     na, nc, ncut = dm.counts()
     ona, onc, oncut = ox.counts()
