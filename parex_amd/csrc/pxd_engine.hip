@@ -609,11 +609,12 @@ struct pxd_engine {
         return v;
     }
     // degree rule (measured on one MI355X): very sparse rows take 2 lanes
-    // each, up to 24 arcs per row the LDS window with 2 lanes, longer rows
-    // about four arcs per lane
+    // each; up to 24 arcs per row the LDS window with 2 lanes once the list
+    // fills the chip with 4096-entry chunks (2^20 entries: 256 chunks, one
+    // workgroup per CU), else 4 lanes; longer rows about four arcs per lane
     int default_variant() const {
         if (avg_deg <= 6.0) return WV_LANES2;
-        if (avg_deg <= 24.0) return WV_WIN2;
+        if (avg_deg <= 24.0) return nalist >= (1u << 20) ? WV_WIN2 : lanes_variant(4);
         return lanes_variant(gs);
     }
     // variant for `pass` at the current list length; launch(v) runs candidate v
@@ -678,6 +679,19 @@ struct pxd_engine {
     void launch_link(const uint32_t* list, uint32_t nl) {
         k_cc_link<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root,
                                                                 heavy_thr, d_cnt);
+    }
+    void launch_disable_win(int lanes, uint32_t nl) {
+        constexpr uint32_t CHUNK = 4096;
+        const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(cdiv(nl, CHUNK), 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
+        if (lanes == 1)
+            k_disable_win<1, 4, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
+        else if (lanes == 2)
+            k_disable_win<2, 4, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
+        else
+            k_disable_win<4, 4, 32768, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
@@ -870,7 +884,12 @@ struct pxd_engine {
         na = na_new;
         nc = nc_new;
         if (nl) {
-            PXD_DISPATCH_GS(launch_disable, nl);
+            // the LDS-window kernel when pinned, else for long lists of short
+            // rows (the pass mutates the rows, so it cannot be timed by rerunning)
+            const int wg = walk_force >= WV_WIN1 ? (walk_force == WV_WIN1 ? 1 : walk_force == WV_WIN2 ? 2 : 4)
+                           : (walk_force < 0 && avg_deg <= 24.0 && avg_deg > 6.0 && nl >= (1u << 20)) ? 2 : 0;
+            if (wg) launch_disable_win(wg, nl);
+            else PXD_DISPATCH_GS(launch_disable, nl);
             if (heavy_thr != 0xFFFFFFFFu) {
                 heavy_valid = false;
                 k_heavy_flags<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_alive, d_dint, heavy_thr, d_hcnt);

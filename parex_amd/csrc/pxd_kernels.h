@@ -1561,6 +1561,106 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
     }
 }
 
+// k_disable with an LDS window of the component roots (the layout of
+// k_walk_win: chunks of CH entries of the ascending member list, WIN roots
+// around a chunk staged when its nodes span less than WIN/2, 64 rows per wave
+// in GS rounds).  The per-row compaction is k_disable's: GS lanes load U
+// chunks of GS arcs, then write the kept arcs in order, each to an index at
+// most its own (all already loaded).  The kept count goes back to the owner
+// lane, which updates dInt and y.
+template <int GS, int U, int WIN, int CH>
+__global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t* __restrict__ list,
+                                                     const uint8_t* __restrict__ alive,
+                                                     const uint32_t* __restrict__ row,
+                                                     uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ root,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     uint32_t heavy, uint32_t N) {
+    static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of root quads, lane groups within a wave");
+    __shared__ uint32_t sr[WIN];
+    constexpr uint32_t NPR = 64 / GS;
+    constexpr uint32_t NW = WBS / 64;
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    const uint32_t g = lane / GS, gl = lane % GS;
+    const uint32_t nch = (nl + CH - 1) / CH;
+    const uint32_t xx = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
+    const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
+    const uint32_t cb = min(nch, xx * cpx), ce = min(nch, cb + cpx);
+    for (uint32_t c = cb + lb; c < ce; c += nb) {
+        const uint32_t c0 = c * CH, c1 = min(c0 + CH, nl);
+        const uint32_t uf = list[c0], ul = list[c1 - 1];
+        const bool use = ul - uf < (uint32_t)WIN / 2;
+        uint32_t wb = 0;
+        if (use) {
+            const uint32_t mid = uf / 2 + ul / 2;
+            wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
+            if (N <= (uint32_t)WIN) wb = 0;
+            else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
+            wb &= ~3u;
+            for (uint32_t k = 4 * threadIdx.x; k < (uint32_t)WIN; k += 4 * WBS) {
+                if (wb + k + 3 < N) {
+                    const uint4 t = *(const uint4*)(root + wb + k);
+                    sr[k] = t.x;
+                    sr[k + 1] = t.y;
+                    sr[k + 2] = t.z;
+                    sr[k + 3] = t.w;
+                } else {
+                    for (uint32_t q = 0; q < 4; ++q)
+                        if (wb + k + q < N) sr[k + q] = root[wb + k + q];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t ib = c0 + w * 64; ib < c1; ib += NW * 64) {
+            const uint32_t i = ib + lane;
+            const bool ok = i < c1;
+            const uint32_t u = ok ? list[i] : list[c0];
+            const uint32_t st = row[u];
+            const uint32_t dd = dint[u];
+            const bool mine = ok && alive[u] && dd <= heavy;   // long rows: k_disable_heavy
+            const uint32_t d = mine ? dd : 0u;
+            const uint32_t ru = root[u];
+            uint32_t res = 0;
+#pragma unroll
+            for (int r = 0; r < GS; ++r) {
+                const uint32_t s = r * NPR + g;
+                const uint32_t rs = __shfl(st, s), rd = __shfl(d, s), rr = __shfl(ru, s);
+                uint32_t kept = 0;
+                for (uint32_t j0 = 0; j0 < rd; j0 += U * GS) {
+                    uint32_t v[U];
+                    bool keep[U];
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t j = j0 + q * GS + gl;
+                        v[q] = j < rd ? nbr[rs + j] : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t j = j0 + q * GS + gl;
+                        const uint32_t off = v[q] - wb;
+                        keep[q] = j < rd && ((use && off < (uint32_t)WIN) ? sr[off] : root[v[q]]) == rr;
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t j = j0 + q * GS + gl;
+                        const uint64_t bal = group_ballot<GS>(keep[q]);
+                        const uint32_t before = (uint32_t)__popcll(bal & ((1ull << gl) - 1ull));
+                        if (keep[q] && kept + before != j) nbr[rs + kept + before] = v[q];
+                        kept += (uint32_t)__popcll(bal);
+                    }
+                }
+                const uint32_t got = __shfl(kept, (lane % NPR) * GS);
+                if (lane / NPR == (uint32_t)r) res = got;
+            }
+            if (mine && res != d) {
+                dint[u] = res;
+                y[u] = res ? x[u] / (double)res : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // 1 for the alive rows of the list above the threshold
 __global__ void k_heavy_flags(uint32_t n, const uint32_t* __restrict__ list, const uint8_t* __restrict__ alive,
                               const uint32_t* __restrict__ dint, uint32_t heavy, uint32_t* __restrict__ f) {

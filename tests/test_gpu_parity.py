@@ -83,6 +83,9 @@ WALK_VARIANT_GRAPHS = {
     "rmat14": lambda: px.rmat(14, 16, 0.57, 0.19, 0.19, 7),
     # 30,000 nodes in blocks of 3,000: windows that cover a block, chunks that span two
     "sbm30k": lambda: px.sbm(30000, 10, 8 / 2999, 1 / 27000, 3),
+    # above every window size: keys and roots outside the 4-byte windows too
+    "sbm100k": lambda: px.sbm(100000, 40, 12 / 2499, 1 / 97500, 5),
+    "rmat17": lambda: px.rmat(17, 8, 0.57, 0.19, 0.19, 7),
 }
 
 
