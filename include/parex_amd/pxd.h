@@ -152,7 +152,7 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
  * 486-499).  variant 0..5: one row per 2, 4, 8, 16, 32, 64 lanes; 6, 7, 8:
  * the LDS-window kernels with 1, 2, 4 lanes per row; -1 (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
- * (b >= 16) every candidate is launched once on the current state and timed,
+ * (b >= 16) every candidate is launched twice on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive
  * pxd_reset); -2: the degree rule without tuning.  Every variant computes the
  * same bits.  pxd_walk_tuning copies the tables (entries 0..32: walk per
