@@ -76,6 +76,14 @@ int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* nod
                          const uint32_t* edge_counts, const uint32_t* const* ranges_ptrs,
                          const uint32_t* const* edges_ptrs);
 
+/* The reference's input path, readGraph + Graph::finalize + initialize
+ * (lib/utils/graph_io.h:49-84, lib/core/graph.h:147-184, cuda/interface.h:24),
+ * with the CSR built on the device: m undirected edges (fx[i], tx[i]), both
+ * arcs of each, rows ordered by target; loops and duplicates kept as given.
+ * Arrays are host memory; an end >= n is refused (PXD_ERR_ARG).  Below 2^31
+ * arcs. */
+int pxd_initialize_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx);
+
 /* The reference reads sc_threshold / rw_threshold from globals on every step
  * call (cuda/devSweepCut.h:701, cuda/devPartition.h:555, 632); callers that
  * change them between steps update the engine with this call. */

@@ -107,6 +107,7 @@ def pxd() -> C.CDLL:
         lib.pxd_create.argtypes = [C.POINTER(_Params), C.POINTER(C.c_void_p)]
         lib.pxd_destroy.argtypes = [P]
         lib.pxd_initialize.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p]
+        lib.pxd_initialize_edges.argtypes = [P, C.c_uint32, C.c_uint64, _u32p, _u32p]
         lib.pxd_initialize_batch.argtypes = [P, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_void_p),
                                              C.POINTER(C.c_void_p)]
         for f in ("pxd_iterate_random_walk", "pxd_compute_sweep_cuts", "pxd_cut_clusters", "pxd_reset"):
@@ -155,7 +156,7 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
-    "pxd_set_walk_variant", "pxd_walk_tuning",
+    "pxd_set_walk_variant", "pxd_walk_tuning", "pxd_initialize_edges",
     "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
     "pxd_serve_stats",
 ]
@@ -386,6 +387,17 @@ class DeviceManager:
         self._check(self._lib.pxd_initialize(self._h, graph.n, graph.m,
                                              np.ascontiguousarray(graph.ranges, np.uint32),
                                              np.ascontiguousarray(graph.edges, np.uint32)))
+
+    def initialize_edges(self, n: int, fx, tx):
+        """readGraph + Graph::finalize + initialize with the CSR built on the
+        device from the undirected edge list (fx[i], tx[i])."""
+        fx = np.ascontiguousarray(fx, np.uint32)
+        tx = np.ascontiguousarray(tx, np.uint32)
+        if fx.shape != tx.shape:
+            raise ValueError("fx and tx differ in length")
+        self._n, self._m, self._graphs = int(n), int(fx.size), 1
+        self._node_counts = np.array([n], np.uint32)
+        self._check(self._lib.pxd_initialize_edges(self._h, int(n), int(fx.size), fx, tx))
 
     def initialize_batch(self, graphs: Sequence[Graph]):
         G = len(graphs)

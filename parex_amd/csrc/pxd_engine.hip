@@ -21,6 +21,7 @@
 
 #include "parex_amd/pxd.h"
 #include "pxd_kernels.h"
+#include "pxd_ingest.h"
 
 using namespace pxd;
 
@@ -382,20 +383,14 @@ struct pxd_engine {
         M_total = aa / 2;
 
         maxdeg = 0;
-        // host CSR of the whole batch (global node ids)
+        // row offsets of the whole batch on the host (O(N)); the arcs go up as
+        // they are and are checked and rebased to global ids on the device
         std::vector<uint32_t> row(N + 1);
-        std::vector<uint32_t> nbr(A);
         for (uint32_t i = 0; i < G; ++i) {
             const uint32_t nb = h_gnode[i], ab = h_garc[i];
-            for (uint32_t u = 0; u < ncnt[i]; ++u) row[nb + u] = ab + rptr[i][u];
-            for (uint64_t j = 0; j < 2ull * ecnt[i]; ++j) {
-                const uint32_t v = eptr[i][j];
-                if (v >= ncnt[i]) return set_err(PXD_ERR_ARG, "graph %u: arc %llu points to %u >= n", i,
-                                                 (unsigned long long)j, v);
-                nbr[ab + j] = nb + v;
-            }
             for (uint32_t u = 0; u < ncnt[i]; ++u) {
                 if (rptr[i][u + 1] < rptr[i][u]) return set_err(PXD_ERR_ARG, "graph %u: ranges decrease at %u", i, u);
+                row[nb + u] = ab + rptr[i][u];
                 maxdeg = std::max(maxdeg, rptr[i][u + 1] - rptr[i][u]);
             }
         }
@@ -404,13 +399,67 @@ struct pxd_engine {
         CHK(allocate());
         const size_t n1 = (size_t)N + 1;
 
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
         HIPC(hipMemcpyAsync(d_row, row.data(), n1 * 4, hipMemcpyHostToDevice, st));
-        if (A) HIPC(hipMemcpyAsync(d_nbr, nbr.data(), (size_t)A * 4, hipMemcpyHostToDevice, st));
+        for (uint32_t i = 0; i < G; ++i) {
+            const uint64_t arcs = 2ull * ecnt[i];
+            if (!arcs) continue;
+            HIPC(hipMemcpyAsync(d_nbr + h_garc[i], eptr[i], arcs * 4, hipMemcpyHostToDevice, st));
+            k_rebase_check<<<grid_for(arcs, BS), BS, 0, st>>>(arcs, d_nbr + h_garc[i], ncnt[i], h_gnode[i],
+                                                               &d_cnt->admit_bad);
+            HIPC(hipGetLastError());
+        }
+        HIPC(hipMemcpyAsync(d_gnode, h_gnode.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
+        uint32_t bad = 0;
+        CHK(read_u32(&d_cnt->admit_bad, &bad));   // also: the host vectors go out of scope after the sync
+        if (bad) {
+            for (uint32_t i = 0; i < G; ++i)
+                for (uint64_t j = 0; j < 2ull * ecnt[i]; ++j)
+                    if (eptr[i][j] >= ncnt[i])
+                        return set_err(PXD_ERR_ARG, "graph %u: arc %llu points to %u >= n", i,
+                                       (unsigned long long)j, eptr[i][j]);
+            return set_err(PXD_ERR_STATE, "arc check failed");
+        }
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
         if (P.keep_original && A)
             HIPC(hipMemcpyAsync(d_nbr0, d_nbr, (size_t)A * 4, hipMemcpyDeviceToDevice, st));
-        HIPC(hipMemcpyAsync(d_gnode, h_gnode.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
+        CHK(sync());
+        return init_state();
+    }
+
+    // one graph from an undirected edge list, CSR built on the device
+    int setup_edges(uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx) {
+        release();
+        serving = false;
+        HIPC(hipSetDevice(dev));
+        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+        if (n >= 0x7FFFFFFFu || 2 * m >= 0xFFFFFFFFull)
+            return set_err(PXD_ERR_ARG, "graph too large for 32-bit indices (%u nodes, %llu edges)", n,
+                           (unsigned long long)m);
+        if (m && (!fx || !tx)) return set_err(PXD_ERR_ARG, "null edge arrays");
+        G = 1;
+        h_gnode.assign({0u, n});
+        h_garc.assign({0u, (uint32_t)(2 * m)});
+        h_gm.assign(1, (uint32_t)m);
+        N = n;
+        A = (uint32_t)(2 * m);
+        M_total = m;
+        CHK(allocate());
+        char msg[256] = {0};
+        const int rc = pxd_ingest_csr(n, m, fx, tx, d_row, d_nbr, st, msg, sizeof msg);
+        if (rc) {
+            h_gnode.clear();
+            return set_err(rc == 1 ? PXD_ERR_ARG : PXD_ERR_HIP, "%s", msg);
+        }
+        std::vector<uint32_t> row((size_t)N + 1);
+        HIPC(hipMemcpyAsync(row.data(), d_row, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost, st));
         HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
-        CHK(sync());  // host vectors go out of scope
+        HIPC(hipMemcpyAsync(d_gnode, h_gnode.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
+        if (P.keep_original && A)
+            HIPC(hipMemcpyAsync(d_nbr0, d_nbr, (size_t)A * 4, hipMemcpyDeviceToDevice, st));
+        CHK(sync());
+        maxdeg = 0;
+        for (uint32_t u = 0; u < N; ++u) maxdeg = std::max(maxdeg, row[u + 1] - row[u]);
         return init_state();
     }
 
@@ -1356,6 +1405,11 @@ int pxd_initialize(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges
     const uint32_t* r[1] = {ranges};
     const uint32_t* ed[1] = {edges};
     return e->setup(1, &n, &m, r, ed);
+}
+
+int pxd_initialize_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx) {
+    if (!e) return set_err(PXD_ERR_ARG, "null engine");
+    return e->setup_edges(n, m, fx, tx);
 }
 
 int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* node_counts, const uint32_t* edge_counts,

@@ -149,6 +149,20 @@ __global__ void k_rebase(uint64_t n, uint32_t* __restrict__ nbr, uint32_t base) 
         nbr[j] += base;
 }
 
+// arcs of one graph of a batch, uploaded as they are: check every arc below n,
+// then local -> global node ids
+__global__ void k_rebase_check(uint64_t arcs, uint32_t* __restrict__ nbr, uint32_t n, uint32_t base,
+                               uint32_t* __restrict__ bad) {
+    uint32_t b = 0u;
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < arcs; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = nbr[j];
+        b |= v >= n ? 1u : 0u;
+        nbr[j] = v + base;
+    }
+    b = group_or<WAVE>(b);
+    if (b && lane_id() == 0) atomicOr(bad, b);
+}
+
 // the admitted graph's single cluster, appended as cluster c at positions
 // [na, na + n), and its per-graph state
 __global__ void k_admit_cluster(uint32_t c, uint32_t na, uint32_t n, uint32_t slot, uint32_t base, uint32_t vol,

@@ -105,3 +105,10 @@ float erfcinvf(float yf) {
     }
     return (float)x;
 }
+
+// the device CSR build uses rocPRIM, which the emulator does not carry
+int pxd_ingest_csr(uint32_t, uint64_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, hipStream_t,
+                   char* err, size_t errlen) {
+    snprintf(err, errlen, "device CSR build not available in the CPU emulation");
+    return 2;
+}

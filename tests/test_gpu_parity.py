@@ -313,3 +313,75 @@ def test_batch_equals_solo():
         assert its[i] == sit, f"graph {i}: iterations {its[i]} vs solo {sit}"
         assert ks[i] == sk and np.array_equal(ids[off:off + g.n], sids), f"graph {i}: partition differs"
         off += g.n
+
+
+def test_initialize_checks_arcs_on_the_device():
+    """Arcs are uploaded as they are and checked (and rebased) on the device:
+    an arc past n is refused with the graph and arc named, and the engine
+    takes a valid input afterwards."""
+    g = px.sbm(600, 5, 0.08, 0.004, 9)
+    edges = g.edges.copy()
+    edges[17] = g.n + 3
+    dm = px.DeviceManager()
+    with pytest.raises(px.PxdError, match="graph 1: arc 17 points to"):
+        dm.initialize_batch([px.sbm(300, 3, 0.06, 0.003, 100), px.Graph(g.n, g.m, g.ranges.copy(), edges)])
+    dm.initialize(g)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    assert dm.expanderDecomposition() == ox.run()
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids)
+
+
+def _edge_list(g):
+    """Each undirected edge once (u < v for the generators' simple graphs)."""
+    src = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(g.ranges))
+    keep = src < g.edges
+    return src[keep], g.edges[keep]
+
+
+@pytest.mark.parametrize("name", ["sbm_c1", "rmat14", "hub", "loops_dups", "empty", "isolated_tail"])
+def test_initialize_edges_builds_the_host_csr(name):
+    """The device CSR build gives the host build's ranges and arcs
+    (lib/core/graph.h:147-184: both arcs of every edge, rows in target
+    order, loops and duplicates kept), and the same decomposition."""
+    if name == "sbm_c1":
+        g = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
+        fx, tx = _edge_list(g)
+    elif name == "rmat14":
+        g = px.rmat(14, 16, 0.57, 0.19, 0.19, 7)
+        fx, tx = _edge_list(g)
+    elif name == "hub":
+        g = hub(3000)
+        fx, tx = _edge_list(g)
+    elif name == "loops_dups":
+        rng = np.random.default_rng(5)
+        fx = rng.integers(0, 300, 2000).astype(np.uint32)
+        tx = rng.integers(0, 300, 2000).astype(np.uint32)
+        fx[:10] = tx[:10]                       # loops
+        fx[10:20], tx[10:20] = fx[20:30], tx[20:30]   # duplicates
+        g = px.from_edges(300, fx, tx)
+    elif name == "empty":
+        fx = tx = np.zeros(0, np.uint32)
+        g = px.from_edges(50, fx, tx)
+    else:
+        fx, tx = np.arange(99, dtype=np.uint32), np.arange(1, 100, dtype=np.uint32)
+        g = px.from_edges(130, fx, tx)       # nodes 100..129 have no arcs
+    dm = px.DeviceManager()
+    dm.initialize_edges(g.n, fx, tx)
+    d = dm.downloadGraph()
+    assert np.array_equal(d.ranges, g.ranges) and np.array_equal(d.edges, g.edges)
+    it = dm.expanderDecomposition()
+    ids, k = dm.getFinalPartition()
+    ref = _engine(g)
+    assert ref.expanderDecomposition() == it
+    rids, rk = ref.getFinalPartition()
+    assert rk == k and np.array_equal(rids, ids)
+
+
+def test_initialize_edges_refuses_an_end_past_n():
+    dm = px.DeviceManager()
+    with pytest.raises(px.PxdError, match="edge 3 .* has an end >= n"):
+        dm.initialize_edges(10, [0, 1, 2, 3], [1, 2, 3, 10])
+    dm.initialize_edges(10, [0, 1, 2, 3], [1, 2, 3, 9])
+    assert dm.expanderDecomposition() >= 1
