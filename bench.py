@@ -94,10 +94,32 @@ def pinned_pool(pool):
     return out, keep
 
 
+def edge_lists(pool):
+    """Each pool graph as an undirected edge list (every edge once) in pinned
+    host memory: requests of clients that send edges, not a CSR."""
+    import torch
+    keep, out = [], {}
+    for kind, graphs in pool.items():
+        out[kind] = []
+        for g in graphs:
+            src = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(g.ranges))
+            sel = src < g.edges
+            arrs = []
+            for a in (src[sel], g.edges[sel]):
+                t = torch.empty(max(a.size, 1), dtype=torch.int32, pin_memory=True)
+                keep.append(t)
+                v = t.numpy().view(np.uint32)[:a.size]
+                v[:] = a
+                arrs.append(v)
+            out[kind].append((g.n, arrs[0], arrs[1]))
+    return out, keep
+
+
 def serve_workload(a):
     rng = np.random.default_rng(13)
     t = time.perf_counter()
     pool, _pinned = pinned_pool(serve_pool())
+    epool, _epinned = edge_lists(pool) if a.admit == "edges" else (None, None)
     log(f"serving pool: {sum(len(v) for v in pool.values())} graphs ({time.perf_counter() - t:.1f} s)")
     rate, dur = 200.0, a.duration
     gaps = rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 100)
@@ -109,6 +131,20 @@ def serve_workload(a):
     log(f"{len(req)} requests over {dur:.0f} s, offered {offered:.3e} edges/s")
     srv = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=4096,
                     params=px.Params(record_node_data=False))
+    if epool is not None:
+        # requests as edge lists: the CSR of each is built on the device at admission
+        srv.enable_edges(max(g.n for v in pool.values() for g in v), max(g.m for v in pool.values() for g in v))
+        eidx = {}
+        for kind, graphs in pool.items():
+            for g, e in zip(graphs, epool[kind]):
+                eidx[id(g)] = e
+
+        def admit(j, tag):
+            n, fx, tx = eidx[id(req[j])]
+            return srv.admit_edges(n, fx, tx, tag)
+    else:
+        def admit(j, tag):
+            return srv.admit(req[j], tag)
     # untimed warm-up: one graph of each kind
     for i, k in enumerate(["sbm", "torus", "rmat"]):
         srv.admit(pool[k][0], 10 ** 9 + i)
@@ -127,7 +163,7 @@ def serve_workload(a):
             queue.append(i)
             i += 1
         ta = time.perf_counter()
-        while queue and srv.admit(req[queue[0]], queue[0]):
+        while queue and admit(queue[0], queue[0]):
             queue.popleft()
         tb = time.perf_counter()
         live = srv.stats()["live_graphs"]
@@ -194,6 +230,7 @@ def serve_workload(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "serving: Poisson 200 req/s, 60% SBM 10K-200K nodes, 30% torus 64^2-1024^2, "
                                "10% RMAT scale 14-21 ef 16; requests drawn from a pool of 72 graphs",
+                   "admission": "edge lists, CSR built on the device" if a.admit == "edges" else "CSR",
                    "requests": len(req), "duration_s": dur, "seed": 13, "offered_edges_per_s": offered},
         "p50_ms": p50, "p99_ms": p99, "mean_in_flight": inflight_sum / max(steps, 1),
         "wall_s": wall, "results_equal_solo": same,
@@ -265,6 +302,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0, help="serving workload: seconds of arrivals")
+    ap.add_argument("--admit", choices=["csr", "edges"], default="csr",
+                    help="serving workload: requests as CSR or as edge lists (CSR built on the device)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",

@@ -189,6 +189,14 @@ int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t 
  * that fails comes back as a result with num_clusters = -1. */
 int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
                     uint64_t tag, int32_t* slot);
+/* Edge-list admissions (the CSR built on the device, as pxd_initialize_edges
+ * does): enable once with the largest graph to expect (a device workspace of
+ * about 40 B per edge and 4 B per node), then admit undirected edge lists
+ * (fx[i], tx[i]) like CSRs -- same PXD_FULL rule, buffers valid until the next
+ * pxd_serve_step; an end >= n comes back as a result with num_clusters = -1. */
+int pxd_serve_enable_edges(pxd_engine* e, uint32_t max_nodes, uint64_t max_edges);
+int pxd_serve_admit_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx,
+                          uint64_t tag, int32_t* slot);
 /* One iteration (walk, sweep, cut) of every graph in flight; *finished =
  * graphs that completed in it (their results wait for pxd_serve_take). */
 int pxd_serve_step(pxd_engine* e, uint32_t* finished);

@@ -138,6 +138,9 @@ def pxd() -> C.CDLL:
         lib.pxd_serve_admit.argtypes = [P, C.c_uint32, C.c_uint32, _u32p, _u32p, C.c_uint64,
                                         C.POINTER(C.c_int32)]
         lib.pxd_serve_step.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.pxd_serve_enable_edges.argtypes = [P, C.c_uint32, C.c_uint64]
+        lib.pxd_serve_admit_edges.argtypes = [P, C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_uint64,
+                                              C.POINTER(C.c_int32)]
         lib.pxd_serve_next.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
         lib.pxd_serve_take.argtypes = [P, _i32p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         lib.pxd_serve_wait.argtypes = [P]
@@ -156,7 +159,8 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
-    "pxd_set_walk_variant", "pxd_walk_tuning", "pxd_initialize_edges",
+    "pxd_set_walk_variant", "pxd_walk_tuning", "pxd_initialize_edges", "pxd_serve_enable_edges",
+    "pxd_serve_admit_edges",
     "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
     "pxd_serve_stats",
 ]
@@ -293,6 +297,7 @@ class Server:
         h = C.c_void_p()
         self._check(self._lib.pxd_serve_create(C.byref(p), node_capacity, arc_capacity, graph_slots, C.byref(h)))
         self._h = h
+        self._keep = []
 
     def _check(self, rc: int):
         if rc < 0:
@@ -318,10 +323,28 @@ class Server:
         rc = self._check(self._lib.pxd_serve_admit(self._h, graph.n, graph.m, ranges, edges, tag, C.byref(slot)))
         return rc == 0
 
+    def enable_edges(self, max_nodes: int, max_edges: int):
+        """Workspace for edge-list admissions of graphs up to these sizes."""
+        self._check(self._lib.pxd_serve_enable_edges(self._h, int(max_nodes), int(max_edges)))
+
+    def admit_edges(self, n: int, fx, tx, tag: int) -> bool:
+        """Admit an undirected edge list (CSR built on the device); True if
+        admitted, False if it does not fit the free capacity now."""
+        fx = np.ascontiguousarray(fx, np.uint32)
+        tx = np.ascontiguousarray(tx, np.uint32)
+        if fx.shape != tx.shape:
+            raise ValueError("fx and tx differ in length")
+        slot = C.c_int32()
+        rc = self._check(self._lib.pxd_serve_admit_edges(self._h, int(n), int(fx.size), fx, tx, tag, C.byref(slot)))
+        if rc == 0:
+            self._keep.append((fx, tx))     # read by the device until the next step
+        return rc == 0
+
     def step(self) -> int:
         """One iteration of every graph in flight; returns how many finished."""
         f = C.c_uint32()
         self._check(self._lib.pxd_serve_step(self._h, C.byref(f)))
+        self._keep = []
         return f.value
 
     def results(self, wait: bool = False):

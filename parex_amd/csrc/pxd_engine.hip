@@ -219,6 +219,10 @@ struct pxd_engine {
         d_hcnt = d_hoff = d_hrow = nullptr;   // heavy-row buffers are allocated lazily
         d_hpart = nullptr;
         heavy_cap = 0;
+        ing_n = 0;
+        ing_m = 0;
+        d_ifx = d_itx = nullptr;
+        ing = IngestWork{};
         inited = false;
     }
 
@@ -1085,6 +1089,12 @@ struct pxd_engine {
         uint32_t base, n, abase, arcs, slot, stage;
     };
     std::vector<Pending> pending;
+    // edge-list admissions: the CSR is built on the device in a workspace for
+    // graphs of up to ing_n nodes and ing_m edges (pxd_serve_enable_edges)
+    uint32_t ing_n = 0;
+    uint64_t ing_m = 0;
+    uint32_t *d_ifx = nullptr, *d_itx = nullptr;
+    IngestWork ing{};
     uint32_t stage_used = 0;          // bump pointer into d_stage (N + G + 1 entries)
     uint32_t* d_stage = nullptr;      // raw ranges of pending graphs
     uint32_t* d_pbad = nullptr;       // device check result per pending graph
@@ -1215,6 +1225,69 @@ struct pxd_engine {
         k_admit_check<<<grid_for(std::max<uint64_t>(n, arcs), BS), BS, 0, st>>>(n, d_stage + stage_used, arcs,
                                                                               d_nbr + ab, d_pbad + k);
         HIPC(hipGetLastError());
+        pending.push_back(Pending{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, g, stage_used});
+        stage_used += n + 1;
+        live_graphs++;
+        live_nodes += n;
+        live_arcs += arcs;
+        *slot_out = (int32_t)g;
+        return PXD_OK;
+    }
+
+    int serve_enable_edges(uint32_t max_n, uint64_t max_m) {
+        if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
+        if (ing_n) return set_err(PXD_ERR_ARG, "edge admissions are already enabled");
+        if (max_n == 0 || max_n > N || 2 * max_m > A || 2 * max_m > 0x7FFFFFFFull)
+            return set_err(PXD_ERR_ARG, "edge-admission limits beyond the serving capacity or 2^31 arcs");
+        const size_t tb = pxd_ingest_tmp_bytes(max_n, 2 * max_m);
+        if (!tb) return set_err(PXD_ERR_HIP, "sizing the device sort failed");
+        CHK(alloc(&d_ifx, max_m));
+        CHK(alloc(&d_itx, max_m));
+        CHK(alloc(&ing.keys_in, 2 * max_m));
+        CHK(alloc(&ing.keys_out, 2 * max_m));
+        CHK(alloc(&ing.deg, (size_t)max_n + 1));
+        char* tmp = nullptr;
+        CHK(alloc(&tmp, tb));
+        ing.tmp = tmp;
+        ing.tmp_bytes = tb;
+        ing_n = max_n;
+        ing_m = max_m;
+        return 0;
+    }
+
+    // admission of an undirected edge list: room as for serve_admit, then the
+    // CSR is built on the device straight into the staged ranges and the arc
+    // range; an end >= n fails the graph's device check like a bad CSR
+    int serve_admit_edges(uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx, uint64_t tag,
+                          int32_t* slot_out) {
+        if (!serving) return set_err(PXD_ERR_ARG, "not a serving engine");
+        if (!ing_n) return set_err(PXD_ERR_ARG, "edge admissions need pxd_serve_enable_edges first");
+        if (!slot_out || (m && (!fx || !tx))) return set_err(PXD_ERR_ARG, "null argument");
+        *slot_out = -1;
+        if (n == 0) {
+            if (m) return set_err(PXD_ERR_ARG, "edges in a graph without nodes");
+            push_done(Done{tag, 0, 1, {}, true, 0u});
+            return PXD_OK;
+        }
+        if (n > ing_n || m > ing_m) return set_err(PXD_ERR_ARG, "graph larger than the edge-admission limits");
+        const uint64_t arcs = 2 * m;
+        if (free_slots.empty()) return PXD_FULL;
+        uint64_t nb = 0, ab = 0;
+        if (!take_range(free_nodes, n, &nb)) return PXD_FULL;
+        if (!take_range(free_arcs, arcs, &ab)) {
+            give_range(free_nodes, nb, n);
+            return PXD_FULL;
+        }
+        const uint32_t g = free_slots.back();
+        free_slots.pop_back();
+        slots[g] = Slot{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, true};
+        const uint32_t k = (uint32_t)pending.size();
+        if (k == 0) HIPC(hipMemsetAsync(d_pbad, 0, G * 4, st));
+        if (m) {
+            HIPC(hipMemcpyAsync(d_ifx, fx, m * 4, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_itx, tx, m * 4, hipMemcpyHostToDevice, st));
+        }
+        HIPC(pxd_ingest_csr_async(n, m, d_ifx, d_itx, d_stage + stage_used, d_nbr + ab, ing, d_pbad + k, st));
         pending.push_back(Pending{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, g, stage_used});
         stage_used += n + 1;
         live_graphs++;
@@ -1670,6 +1743,15 @@ int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t 
 int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
                     uint64_t tag, int32_t* slot) {
     return e ? e->serve_admit(n, m, ranges, edges, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+}
+
+int pxd_serve_enable_edges(pxd_engine* e, uint32_t max_nodes, uint64_t max_edges) {
+    return e ? e->serve_enable_edges(max_nodes, max_edges) : set_err(PXD_ERR_ARG, "null engine");
+}
+
+int pxd_serve_admit_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx, uint64_t tag,
+                          int32_t* slot) {
+    return e ? e->serve_admit_edges(n, m, fx, tx, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
 }
 
 int pxd_serve_step(pxd_engine* e, uint32_t* finished) {

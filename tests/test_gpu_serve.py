@@ -138,3 +138,47 @@ def test_serve_refuses_invalid_graph_and_carries_on():
     drain(srv, [0], [good], got)
     srv.close()
     check([good], got)
+
+
+def _edges(g):
+    src = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(g.ranges))
+    keep = src < g.edges
+    return src[keep], g.edges[keep]
+
+
+def test_serve_edge_lists_equal_solo_and_csr_admissions():
+    """Edge-list admissions (CSR built on the device, no host sync per
+    admission) mixed with CSR admissions, joining while others run: every
+    result equals the graph's solo run; an edge past n fails that graph only."""
+    graphs = mix()
+    ref = [solo(g) for g in graphs]
+    srv = px.Server(node_capacity=20000, arc_capacity=400000, graph_slots=16,
+                    params=px.Params(record_node_data=False))
+    srv.enable_edges(5000, 60000)
+    got = {}
+    bad_fx, bad_tx = np.array([0, 1, 2], np.uint32), np.array([1, 2, 9], np.uint32)
+    for rnd in range(2):
+        for i, g in enumerate(graphs):
+            tag = rnd * 100 + i
+            if (i + rnd) % 2 == 0:
+                assert srv.admit_edges(g.n, *_edges(g), tag)
+            else:
+                assert srv.admit(g, tag)
+            if i == 3 and rnd == 0:
+                assert srv.admit_edges(5, bad_fx, bad_tx, 999)
+            srv.step()
+            for t, ids, k, it in srv.results():
+                got[t] = (ids, k, it)
+    while srv.stats()["live_graphs"]:
+        srv.step()
+    for t, ids, k, it in srv.results(wait=True):
+        got[t] = (ids, k, it)
+    assert got[999][1] == -1
+    for rnd in range(2):
+        for i in range(len(graphs)):
+            ids, k, it = got[rnd * 100 + i]
+            rids, rk, rit = ref[i]
+            assert k == rk and it == rit and np.array_equal(ids, rids), f"graph {i} round {rnd} differs from solo"
+    with pytest.raises(px.PxdError, match="edge-admission limits"):
+        srv.admit_edges(6000, [0], [1], 5)
+    srv.close()
