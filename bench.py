@@ -357,7 +357,8 @@ def main():
                 "alg_bytes_per_launch": round(walk_bytes / max(launches, 1)),
                 # walk kernel chosen per log2(active-list length) class by the
                 # online tuner (first decomposition run): 0..5 = 2..64 lanes
-                # per row, 6/7/8 = LDS window with 1/2/4 lanes per row
+                # per row, 6/7/8 = LDS window with 1/2/4 lanes per row, 9 = LDS window,
+                # 2 lanes, 1024-entry chunks
                 "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
                 "tunings": tunings}
     dm.close()

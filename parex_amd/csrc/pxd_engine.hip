@@ -185,14 +185,15 @@ struct pxd_engine {
     // kernel choice for the two arc passes of an iteration (0 = walk, 1 = cut
     // contribution): variants 0..5 = one row per 2, 4, ..., 64 lanes
     // (k_walk / k_contrib), 6..8 = the LDS-window kernels with 1, 2, 4 lanes
-    // per row.  walk_force >= 0 pins one variant for both passes; -1 = online
+    // per row (4096-entry chunks), 9 = 2 lanes per row, 1024-entry chunks and
+    // a quarter of the window.  walk_force >= 0 pins one variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
     // every candidate is launched twice on the current state (both passes are
     // pure functions of it, all variants give the same bits) and timed, and
     // the fastest is kept for that pass and length class; -2 = the degree rule
     // without tuning.  The tables survive pxd_reset, so repeated
     // decompositions of the same input reuse them.
-    static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_COUNT = 9;
+    static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_COUNT = 10;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     int tuned[2][33];
@@ -562,6 +563,11 @@ struct pxd_engine {
             HIPC(hipGetLastError());
             return 0;
         }
+        if (n > BS * SCAN_ITEMS && n <= SMALL_SCAN_N) {
+            k_scan_small<<<1, 1024, 0, st>>>(n, in, out);
+            HIPC(hipGetLastError());
+            return 0;
+        }
         return scan_level(in, out, n, 0);
     }
 
@@ -653,6 +659,7 @@ struct pxd_engine {
             case 5: launch_walk<64>(); break;
             case WV_WIN1: launch_walk_win<1, 8, 20480, 4096>(); break;
             case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
+            case WV_WIN2S: launch_walk_win<2, 8, 8192, 1024>(); break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
     }
@@ -684,6 +691,8 @@ struct pxd_engine {
         if (gs <= 8) cand.push_back(WV_WIN2);
         if (gs <= 4) cand.push_back(WV_WIN1);
         if (gs >= 4 && gs <= 8) cand.push_back(WV_WIN4);
+        // short chunks, small window: mid-size lists that 4096-entry chunks leave CUs idle on
+        if (gs <= 8 && nalist < (1u << 22)) cand.push_back(WV_WIN2S);
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -727,6 +736,7 @@ struct pxd_engine {
             case 5: launch_contrib<64>(); break;
             case WV_WIN1: launch_contrib_win<1, 8, 40960, 4096>(); break;
             case WV_WIN2: launch_contrib_win<2, 8, 40960, 4096>(); break;
+            case WV_WIN2S: launch_contrib_win<2, 8, 16384, 1024>(); break;
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }
     }
@@ -941,7 +951,8 @@ struct pxd_engine {
         if (nl) {
             // the LDS-window kernel when pinned, else for long lists of short
             // rows (the pass mutates the rows, so it cannot be timed by rerunning)
-            const int wg = walk_force >= WV_WIN1 ? (walk_force == WV_WIN1 ? 1 : walk_force == WV_WIN2 ? 2 : 4)
+            const int wg = walk_force >= WV_WIN1
+                               ? (walk_force == WV_WIN1 ? 1 : walk_force == WV_WIN4 ? 4 : 2)
                            : (walk_force < 0 && avg_deg <= 24.0 && avg_deg > 6.0 && nl >= (1u << 20)) ? 2 : 0;
             if (wg) launch_disable_win(wg, nl);
             else PXD_DISPATCH_GS(launch_disable, nl);

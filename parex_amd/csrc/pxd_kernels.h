@@ -660,6 +660,31 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == BS - 1) out[n] = run;
 }
 
+// Exclusive scan of up to SMALL_SCAN_N elements in one workgroup (one launch
+// instead of the three of the two-level scan); out[n] = total.
+constexpr uint32_t SMALL_SCAN_ITEMS = 16;
+constexpr uint32_t SMALL_SCAN_N = 1024 * SMALL_SCAN_ITEMS;
+__global__ void __launch_bounds__(1024) k_scan_small(uint32_t n, const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out) {
+    __shared__ uint32_t lds[1024 / WAVE];
+    const uint32_t base = threadIdx.x * SMALL_SCAN_ITEMS;
+    uint32_t v[SMALL_SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < SMALL_SCAN_ITEMS; ++i) {
+        v[i] = base + i < n ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<1024>(s, lds, tot);
+#pragma unroll
+    for (uint32_t i = 0; i < SMALL_SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == 0) out[n] = tot;
+}
+
 // ------------------------------------------------------------------ segmented sort
 
 // Sort classes: size <= 1 nothing, <= 64 one wave, <= TILE one block in LDS,
