@@ -917,6 +917,7 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
                                                 const uint32_t* __restrict__ cs,
                                                 const int32_t* __restrict__ ediff,
                                                 const uint32_t* __restrict__ dint,
+                                                uint32_t* __restrict__ ped, uint32_t* __restrict__ pvol,
                                                 uint32_t* __restrict__ tail_ed, uint32_t* __restrict__ tail_vol,
                                                 uint32_t* __restrict__ has_start) {
     __shared__ uint32_t s_ed, s_vol, s_start;
@@ -931,10 +932,14 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
         if (k >= b1) break;
         const uint32_t c = pclu[k];
         if (cs[c] == k) st = 1u;
+        // the node's values gathered once, kept in position order for k_sweep_c
+        const uint32_t u = pnix[k];
+        const uint32_t e = (uint32_t)ediff[u], d = dint[u];
+        ped[k] = e;
+        pvol[k] = d;
         if (c == lastc) {
-            const uint32_t u = pnix[k];
-            ed += (uint32_t)ediff[u];
-            vol += dint[u];
+            ed += e;
+            vol += d;
         }
     }
     ed = group_sum<WAVE>(ed);
@@ -996,8 +1001,8 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
                                                 const uint32_t* __restrict__ pclu,
                                                 const uint32_t* __restrict__ cs,
                                                 const uint32_t* __restrict__ cvol,
-                                                const int32_t* __restrict__ ediff,
-                                                const uint32_t* __restrict__ dint,
+                                                const uint32_t* __restrict__ ped,
+                                                const uint32_t* __restrict__ pvol,
                                                 const uint32_t* __restrict__ carry_ed,
                                                 const uint32_t* __restrict__ carry_vol,
                                                 unsigned long long* __restrict__ cbest,
@@ -1014,10 +1019,10 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
         ok[i] = k < na;
         if (ok[i]) {
             cc[i] = pclu[k];
-            uu[i] = pnix[k];
+            uu[i] = nd ? pnix[k] : 0u;
             st[i] = cs[cc[i]] == k;
-            ed[i] = (uint32_t)ediff[uu[i]];
-            vo[i] = dint[uu[i]];
+            ed[i] = ped[k];
+            vo[i] = pvol[k];
         } else {
             cc[i] = 0u; uu[i] = 0u; st[i] = true; ed[i] = 0u; vo[i] = 0u;
         }
