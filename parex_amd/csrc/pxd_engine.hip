@@ -187,14 +187,16 @@ struct pxd_engine {
     // contribution): variants 0..5 = one row per 2, 4, ..., 64 lanes
     // (k_walk / k_contrib), 6..8 = the LDS-window kernels with 1, 2, 4 lanes
     // per row (4096-entry chunks), 9 = 2 lanes per row, 1024-entry chunks and
-    // a quarter of the window.  walk_force >= 0 pins one variant for both passes; -1 = online
+    // a quarter of the window, 10 = variant 7 with clamped slots (walk only;
+    // the cut contribution runs variant 7 for it).  walk_force >= 0 pins one variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
     // every candidate is launched twice on the current state (both passes are
     // pure functions of it, all variants give the same bits) and timed, and
     // the fastest is kept for that pass and length class; -2 = the degree rule
     // without tuning.  The tables survive pxd_reset, so repeated
     // decompositions of the same input reuse them.
-    static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_COUNT = 10;
+    static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_WIN2C = 10,
+                         WV_COUNT = 11;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     int tuned[2][33];
@@ -644,11 +646,11 @@ struct pxd_engine {
                                                                          d_y[cur], d_x[cur], d_x[cur ^ 1],
                                                                          d_y[cur ^ 1], d_xkey, heavy_thr, d_cnt);
     }
-    template <int GSZ, int U, int WIN, int CHUNK>
+    template <int GSZ, int U, int WIN, int CHUNK, bool CLAMP = false>
     void launch_walk_win() {
         const uint32_t nch = cdiv(nalist, CHUNK);
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
-        k_walk_win<GSZ, U, WIN, CHUNK><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur],
+        k_walk_win<GSZ, U, WIN, CHUNK, CLAMP><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur],
                                                             d_x[cur], d_x[cur ^ 1], d_y[cur ^ 1], d_xkey, heavy_thr,
                                                             N, d_cnt);
     }
@@ -663,6 +665,7 @@ struct pxd_engine {
             case WV_WIN1: launch_walk_win<1, 8, 20480, 4096>(); break;
             case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
             case WV_WIN2S: launch_walk_win<2, 8, 8192, 1024>(); break;
+            case WV_WIN2C: launch_walk_win<2, 8, 20480, 4096, true>(); break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
     }
@@ -696,6 +699,8 @@ struct pxd_engine {
         if (gs >= 4 && gs <= 8) cand.push_back(WV_WIN4);
         // short chunks, small window: mid-size lists that 4096-entry chunks leave CUs idle on
         if (gs <= 8 && nalist < (1u << 22)) cand.push_back(WV_WIN2S);
+        // the walk's clamped-slot form, for long lists of rows that fill their slots
+        if (pass == 0 && gs <= 8 && nalist >= (1u << 22)) cand.push_back(WV_WIN2C);
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -740,6 +745,7 @@ struct pxd_engine {
             case WV_WIN1: launch_contrib_win<1, 8, 40960, 4096>(); break;
             case WV_WIN2: launch_contrib_win<2, 8, 40960, 4096>(); break;
             case WV_WIN2S: launch_contrib_win<2, 8, 16384, 1024>(); break;
+            case WV_WIN2C: launch_contrib_win<2, 8, 40960, 4096>(); break;   // no clamped form
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }
     }

@@ -413,7 +413,11 @@ __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, co
 // gathers in flight per lane, each exact row sum goes back to its owner lane,
 // and all 64 lanes round and store together.  Same arithmetic as k_walk, so
 // the same bits.  Rows above `heavy` are left to k_walk_heavy.
-template <int GS, int U, int WIN, int CH>
+// CLAMP: slots past a row's end load the row's last arc again (no exec-mask
+// region per slot), the LDS value is read unconditionally and only window
+// misses branch to a global load; the slot's term is then dropped.  Fewer
+// branch regions when rows fill their slots, more wasted loads when they do not.
+template <int GS, int U, int WIN, int CH, bool CLAMP = false>
 __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* __restrict__ alist,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
@@ -474,16 +478,29 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                 for (uint32_t j = gl; j < rd; j += U * GS) {
                     uint32_t q[U];
                     double yv[U];
+                    if constexpr (CLAMP) {
 #pragma unroll
-                    for (int k = 0; k < U; ++k) {
-                        const uint32_t jj = j + k * GS;
-                        q[k] = jj < rd ? nbr[rs + jj] : 0u;
-                    }
+                        for (int k = 0; k < U; ++k) q[k] = nbr[rs + min(j + k * GS, rd - 1)];
 #pragma unroll
-                    for (int k = 0; k < U; ++k) {
-                        const uint32_t jj = j + k * GS;
-                        const uint32_t off = q[k] - wb;
-                        yv[k] = jj < rd ? ((use && off < (uint32_t)WIN) ? sy[off] : y[q[k]]) : 0.0;
+                        for (int k = 0; k < U; ++k) {
+                            const uint32_t off = q[k] - wb;
+                            const bool inw = use && off < (uint32_t)WIN;
+                            double t = sy[inw ? off : 0u];
+                            if (!inw) t = y[q[k]];
+                            yv[k] = j + k * GS < rd ? t : 0.0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < U; ++k) {
+                            const uint32_t jj = j + k * GS;
+                            q[k] = jj < rd ? nbr[rs + jj] : 0u;
+                        }
+#pragma unroll
+                        for (int k = 0; k < U; ++k) {
+                            const uint32_t jj = j + k * GS;
+                            const uint32_t off = q[k] - wb;
+                            yv[k] = jj < rd ? ((use && off < (uint32_t)WIN) ? sy[off] : y[q[k]]) : 0.0;
+                        }
                     }
 #pragma unroll
                     for (int k = 0; k < U; ++k) fix_add_term(acc, yv[k], ovf);
