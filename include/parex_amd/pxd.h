@@ -160,7 +160,9 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
  * 486-499).  variant 0..5: one row per 2, 4, 8, 16, 32, 64 lanes; 6, 7, 8:
  * the LDS-window kernels with 1, 2, 4 lanes per row; 9: the window kernel
  * with 2 lanes, 1024-entry chunks and a quarter window; 10: variant 7 with
- * clamped slots in the walk (variant 7 for the cut contribution); -1 (default): online
+ * clamped slots in the walk (variant 7 for the cut contribution); 11: the
+ * window kernel with 2 lanes over node-index-range chunks (windows that
+ * cover sparse lists); -1 (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
  * (b >= 16) every candidate is launched twice on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive

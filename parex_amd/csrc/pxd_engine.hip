@@ -188,7 +188,8 @@ struct pxd_engine {
     // (k_walk / k_contrib), 6..8 = the LDS-window kernels with 1, 2, 4 lanes
     // per row (4096-entry chunks), 9 = 2 lanes per row, 1024-entry chunks and
     // a quarter of the window, 10 = variant 7 with clamped slots (walk only;
-    // the cut contribution runs variant 7 for it).  walk_force >= 0 pins one variant for both passes; -1 = online
+    // the cut contribution runs variant 7 for it), 11 = 2 lanes per row over
+    // node-range chunks of RANGE_R indices.  walk_force >= 0 pins one variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
     // every candidate is launched twice on the current state (both passes are
     // pure functions of it, all variants give the same bits) and timed, and
@@ -196,7 +197,10 @@ struct pxd_engine {
     // without tuning.  The tables survive pxd_reset, so repeated
     // decompositions of the same input reuse them.
     static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_WIN2C = 10,
-                         WV_COUNT = 11;
+                         WV_WIN2R = 11, WV_COUNT = 12;
+    static constexpr uint32_t RANGE_R = 8192;        // node-range chunks of variant 11
+    uint32_t* d_rmap = nullptr;                      // k_range_offsets of the current list
+    bool rmap_valid = false;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     int tuned[2][33];
@@ -293,6 +297,7 @@ struct pxd_engine {
         CHK(alloc(&d_xkey, N));
         CHK(alloc(&d_ediff, N));
         CHK(alloc(&d_ped, N));
+        CHK(alloc(&d_rmap, (size_t)N / RANGE_R + 2));
         CHK(alloc(&d_pvol, N));
         CHK(alloc(&d_root, N));
         CHK(alloc(&d_part, N));
@@ -534,6 +539,7 @@ struct pxd_engine {
         // rows longer than 32 arcs per lane leave the lane-group kernels
         heavy_thr = maxdeg > 32u * gs ? 32u * gs : 0xFFFFFFFFu;
         heavy_valid = false;
+        rmap_valid = false;
         if (heavy_thr != 0xFFFFFFFFu && !d_hcnt) {
             heavy_cap = A / HEAVY_CHUNK + A / heavy_thr + 1u;
             CHK(alloc(&d_hcnt, N));
@@ -646,13 +652,21 @@ struct pxd_engine {
                                                                          d_y[cur], d_x[cur], d_x[cur ^ 1],
                                                                          d_y[cur ^ 1], d_xkey, heavy_thr, d_cnt);
     }
-    template <int GSZ, int U, int WIN, int CHUNK, bool CLAMP = false>
+    template <int GSZ, int U, int WIN, int CHUNK, bool CLAMP = false, bool RANGE = false>
     void launch_walk_win() {
-        const uint32_t nch = cdiv(nalist, CHUNK);
+        const uint32_t nch = RANGE ? cdiv(N, CHUNK) : cdiv(nalist, CHUNK);
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
-        k_walk_win<GSZ, U, WIN, CHUNK, CLAMP><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur],
-                                                            d_x[cur], d_x[cur ^ 1], d_y[cur ^ 1], d_xkey, heavy_thr,
-                                                            N, d_cnt);
+        if (RANGE) range_map();
+        k_walk_win<GSZ, U, WIN, CHUNK, CLAMP, RANGE><<<grid, WBS, 0, st>>>(
+            nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur], d_x[cur], d_x[cur ^ 1], d_y[cur ^ 1], d_xkey,
+            heavy_thr, N, d_cnt, d_rmap);
+    }
+    // list entries of the node-range chunks (valid until the list changes)
+    void range_map() {
+        if (rmap_valid) return;
+        const uint32_t nch = cdiv(N, RANGE_R);
+        k_range_offsets<<<grid_for((uint64_t)nch + 1, BS), BS, 0, st>>>(nch, RANGE_R, d_alist[acur], nalist, d_rmap);
+        rmap_valid = true;
     }
     void launch_walk_variant(int v) {
         switch (v) {
@@ -666,6 +680,7 @@ struct pxd_engine {
             case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
             case WV_WIN2S: launch_walk_win<2, 8, 8192, 1024>(); break;
             case WV_WIN2C: launch_walk_win<2, 8, 20480, 4096, true>(); break;
+            case WV_WIN2R: launch_walk_win<2, 8, 20480, RANGE_R, false, true>(); break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
     }
@@ -701,6 +716,8 @@ struct pxd_engine {
         if (gs <= 8 && nalist < (1u << 22)) cand.push_back(WV_WIN2S);
         // the walk's clamped-slot form, for long lists of rows that fill their slots
         if (pass == 0 && gs <= 8 && nalist >= (1u << 22)) cand.push_back(WV_WIN2C);
+        // node-range chunks: windows that cover sparse lists too
+        if (gs <= 8) cand.push_back(WV_WIN2R);
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -727,12 +744,13 @@ struct pxd_engine {
         k_contrib<GSZ><<<grid_xcd(nalist, BS / GSZ), BS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
                                                                      d_xkey, d_ediff, heavy_thr);
     }
-    template <int GSZ, int U, int WIN, int CHUNK>
+    template <int GSZ, int U, int WIN, int CHUNK, bool RANGE = false>
     void launch_contrib_win() {
-        const uint32_t nch = cdiv(nalist, CHUNK);
+        const uint32_t nch = RANGE ? cdiv(N, CHUNK) : cdiv(nalist, CHUNK);
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
-        k_contrib_win<GSZ, U, WIN, CHUNK><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr, d_xkey,
-                                                               d_ediff, heavy_thr, N);
+        if (RANGE) range_map();
+        k_contrib_win<GSZ, U, WIN, CHUNK, RANGE><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
+                                                                      d_xkey, d_ediff, heavy_thr, N, d_rmap);
     }
     void launch_contrib_variant(int v) {
         switch (v) {
@@ -746,6 +764,7 @@ struct pxd_engine {
             case WV_WIN2: launch_contrib_win<2, 8, 40960, 4096>(); break;
             case WV_WIN2S: launch_contrib_win<2, 8, 16384, 1024>(); break;
             case WV_WIN2C: launch_contrib_win<2, 8, 40960, 4096>(); break;   // no clamped form
+            case WV_WIN2R: launch_contrib_win<2, 8, 40960, RANGE_R, true>(); break;
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }
     }
@@ -880,6 +899,7 @@ struct pxd_engine {
         if (!sweep_done) return set_err(PXD_ERR_ARG, "cutClusters needs a preceding computeSweepCuts");
         sweep_done = false;
         heavy_valid = false;          // the cut changes the active list and internal degrees
+        rmap_valid = false;
         ++iter1;
         if (na == 0 || nc == 0) return 0;
         ClusterTable* T = &ct[ctc];
@@ -968,6 +988,7 @@ struct pxd_engine {
             else PXD_DISPATCH_GS(launch_disable, nl);
             if (heavy_thr != 0xFFFFFFFFu) {
                 heavy_valid = false;
+                rmap_valid = false;
                 k_heavy_flags<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_alive, d_dint, heavy_thr, d_hcnt);
                 CHK(scan(d_hcnt, d_hoff, nl));
                 k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_hcnt, d_hoff, d_cclist, d_hrow);
@@ -1010,6 +1031,7 @@ struct pxd_engine {
             const bool heavy = heavy_thr != 0xFFFFFFFFu;
             if (heavy) {              // chunk map of the list's long rows (the walk's map is not needed now)
                 heavy_valid = false;
+                rmap_valid = false;
                 k_heavy_count<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_cclist2, d_dint, heavy_thr, d_hcnt);
                 CHK(scan(d_hcnt, d_hoff, nu));
                 k_heavy_expand<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_hcnt, d_hoff, d_hrow);
@@ -1206,6 +1228,7 @@ struct pxd_engine {
         gs = 4;
         heavy_thr = 128u;
         heavy_valid = false;
+        rmap_valid = false;
         std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
         serving = true;
         inited = true;
@@ -1352,6 +1375,7 @@ struct pxd_engine {
         pending.clear();
         stage_used = 0;
         heavy_valid = false;
+        rmap_valid = false;
         sweep_done = false;
         return 0;
     }
@@ -1368,6 +1392,7 @@ struct pxd_engine {
             if (heavy_thr != 32u * gs) {
                 heavy_thr = 32u * gs;
                 heavy_valid = false;
+                rmap_valid = false;
             }
             CHK(walk());
             CHK(sweep());

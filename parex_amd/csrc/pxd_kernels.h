@@ -292,30 +292,42 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
 // (chunks of CH list entries per workgroup, WIN keys around a chunk staged
 // with coalesced 16-byte loads when its nodes span less than WIN/2, 64 rows
 // per wave in GS rounds, each row's count returned to its owner lane).
-template <int GS, int U, int WIN, int CH>
+template <int GS, int U, int WIN, int CH, bool RANGE = false>
 __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t* __restrict__ alist,
                                                      const uint32_t* __restrict__ row,
                                                      const uint32_t* __restrict__ dint,
                                                      const uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ xkey,
-                                                     int32_t* __restrict__ ediff, uint32_t heavy, uint32_t N) {
+                                                     int32_t* __restrict__ ediff, uint32_t heavy, uint32_t N,
+                                                     const uint32_t* __restrict__ coff = nullptr) {
     static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of key quads, lane groups within a wave");
     __shared__ uint32_t sk[WIN];
     constexpr uint32_t NPR = 64 / GS;
     constexpr uint32_t NW = WBS / 64;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     const uint32_t g = lane / GS, gl = lane % GS;
-    const uint32_t nch = (na + CH - 1) / CH;
+    const uint32_t nch = RANGE ? (N + CH - 1) / CH : (na + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
     const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
     for (uint32_t c = cb + lb; c < ce; c += nb) {
-        const uint32_t c0 = c * CH, c1 = min(c0 + CH, na);
-        const uint32_t uf = alist[c0], ul = alist[c1 - 1];
-        const bool use = ul - uf < (uint32_t)WIN / 2;
+        uint32_t c0, c1, mid;
+        bool use;
+        if constexpr (RANGE) {
+            c0 = coff[c];
+            c1 = coff[c + 1];
+            if (c0 == c1) continue;
+            use = true;
+            mid = c * CH + CH / 2;
+        } else {
+            c0 = c * CH;
+            c1 = min(c0 + CH, na);
+            const uint32_t uf = alist[c0], ul = alist[c1 - 1];
+            use = ul - uf < (uint32_t)WIN / 2;
+            mid = uf / 2 + ul / 2;
+        }
         uint32_t wb = 0;
         if (use) {
-            const uint32_t mid = uf / 2 + ul / 2;
             wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
             if (N <= (uint32_t)WIN) wb = 0;
             else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
@@ -401,6 +413,21 @@ __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, co
     }
 }
 
+// coff[c] = first entry of the ascending node list at or after node c*R, for
+// c = 0..nch (coff[nch] = n): the list entries of the node range chunks
+__global__ void k_range_offsets(uint32_t nch, uint32_t R, const uint32_t* __restrict__ alist, uint32_t n,
+                                uint32_t* __restrict__ coff) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nch; c += gridDim.x * blockDim.x) {
+        const uint64_t key = (uint64_t)c * R;
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t m = lo + (hi - lo) / 2;
+            if ((uint64_t)alist[m] < key) lo = m + 1; else hi = m;
+        }
+        coff[c] = c == nch ? n : lo;
+    }
+}
+
 // Walk with an LDS window of y.  A workgroup of WBS threads takes chunks of CH
 // consecutive node-list entries.  When a chunk's nodes span less than WIN/2
 // indices, the WIN values of y around them are staged into LDS with coalesced
@@ -417,7 +444,12 @@ __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, co
 // region per slot), the LDS value is read unconditionally and only window
 // misses branch to a global load; the slot's term is then dropped.  Fewer
 // branch regions when rows fill their slots, more wasted loads when they do not.
-template <int GS, int U, int WIN, int CH, bool CLAMP = false>
+//
+// RANGE: chunks are node-index ranges [c*CH, (c+1)*CH) instead of CH list
+// entries; coff[c] (k_range_offsets) is the first list entry of chunk c.  The
+// window always covers its chunk, however sparse the list (a serving engine's
+// node space, late iterations); chunks without active nodes are skipped.
+template <int GS, int U, int WIN, int CH, bool CLAMP = false, bool RANGE = false>
 __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* __restrict__ alist,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
@@ -425,7 +457,8 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                                                   const double* __restrict__ y,
                                                   const double* __restrict__ xc, double* __restrict__ xn,
                                                   double* __restrict__ yn, uint32_t* __restrict__ xkey,
-                                                  uint32_t heavy, uint32_t N, Counters* cnt) {
+                                                  uint32_t heavy, uint32_t N, Counters* cnt,
+                                                  const uint32_t* __restrict__ coff = nullptr) {
     static_assert(WIN % 2 == 0 && 64 % GS == 0, "window of double pairs, lane groups within a wave");
     __shared__ double sy[WIN];
     constexpr uint32_t NPR = 64 / GS;           // rows per round
@@ -433,18 +466,29 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     const uint32_t g = lane / GS, gl = lane % GS;
     // XCD-aware chunk split (see xcd_span): XCD x takes the x-th eighth of the chunks
-    const uint32_t nch = (na + CH - 1) / CH;
+    const uint32_t nch = RANGE ? (N + CH - 1) / CH : (na + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
     const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
     uint32_t ovf = 0;
     for (uint32_t c = cb + lb; c < ce; c += nb) {
-        const uint32_t c0 = c * CH, c1 = min(c0 + CH, na);
-        const uint32_t uf = alist[c0], ul = alist[c1 - 1];   // the list is ascending
-        const bool use = ul - uf < (uint32_t)WIN / 2;
+        uint32_t c0, c1, mid;
+        bool use;
+        if constexpr (RANGE) {
+            c0 = coff[c];
+            c1 = coff[c + 1];
+            if (c0 == c1) continue;   // no active node in the range (uniform over the workgroup)
+            use = true;
+            mid = c * CH + CH / 2;
+        } else {
+            c0 = c * CH;
+            c1 = min(c0 + CH, na);
+            const uint32_t uf = alist[c0], ul = alist[c1 - 1];   // the list is ascending
+            use = ul - uf < (uint32_t)WIN / 2;
+            mid = uf / 2 + ul / 2;
+        }
         uint32_t wb = 0;
         if (use) {
-            const uint32_t mid = uf / 2 + ul / 2;
             wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
             if (N <= (uint32_t)WIN) wb = 0;
             else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
