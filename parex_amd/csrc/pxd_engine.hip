@@ -724,9 +724,9 @@ struct pxd_engine {
             for (int rep = 0; rep < 2; ++rep) {          // the faster of two launches
                 if (hipEventRecord(tev0, st) != hipSuccess) return def;
                 launch(v);
-                if (hipGetLastError() != hipSuccess || hipEventRecord(tev1, st) != hipSuccess ||
-                    hipEventSynchronize(tev1) != hipSuccess)
-                    return def;
+                if (hipGetLastError() != hipSuccess) break;   // a candidate that cannot launch drops out
+                if (hipEventRecord(tev1, st) != hipSuccess || hipEventSynchronize(tev1) != hipSuccess)
+                    return def;                               // a failed stream shows at the caller's launch
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, tev0, tev1) != hipSuccess) return def;
                 if (best_ms == 0.f || ms < best_ms) {
