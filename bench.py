@@ -131,6 +131,8 @@ def serve_workload(a):
     log(f"{len(req)} requests over {dur:.0f} s, offered {offered:.3e} edges/s")
     srv = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=4096,
                     params=px.Params(record_node_data=False))
+    if a.walk_variant is not None:
+        srv.set_walk_variant(a.walk_variant)
     if epool is not None:
         # requests as edge lists: the CSR of each is built on the device at admission
         srv.enable_edges(max(g.n for v in pool.values() for g in v), max(g.m for v in pool.values() for g in v))
@@ -193,6 +195,7 @@ def serve_workload(a):
             last_log = time.perf_counter()
             log(f"  t={last_log - t0:.0f} s: {finished}/{len(req)} done, {live} in flight, {len(queue)} queued")
     wall = time.perf_counter() - t0
+    srv_tuning = srv.walk_tuning()
     # results in the mix equal solo results (sample, untimed)
     same = True
     probe = rng.choice(len(req), size=min(12, len(req)), replace=False)
@@ -234,6 +237,7 @@ def serve_workload(a):
                    "requests": len(req), "duration_s": dur, "seed": 13, "offered_edges_per_s": offered},
         "p50_ms": p50, "p99_ms": p99, "mean_in_flight": inflight_sum / max(steps, 1),
         "wall_s": wall, "results_equal_solo": same,
+        "walk_variant": a.walk_variant, "tuning": dict(zip(("walk", "contrib", "tunings"), srv_tuning)),
         "host_seconds": {k: round(v, 3) for k, v in tsp.items()},
         "roofline": None, "cpu_baseline": None,
     }
@@ -302,6 +306,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0, help="serving workload: seconds of arrivals")
+    ap.add_argument("--walk-variant", type=int, default=None,
+                    help="serving workload: pin the per-arc kernel variant (default: online tuning)")
     ap.add_argument("--admit", choices=["csr", "edges"], default="csr",
                     help="serving workload: requests as CSR or as edge lists (CSR built on the device)")
     a = ap.parse_args()

@@ -162,7 +162,8 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
  * with 2 lanes, 1024-entry chunks and a quarter window; 10: variant 7 with
  * clamped slots in the walk (variant 7 for the cut contribution); 11: the
  * window kernel with 2 lanes over node-index-range chunks (windows that
- * cover sparse lists); -1 (default): online
+ * cover sparse lists; it needs the node list in index order, so a serving
+ * engine, whose admitted graphs are appended, runs variant 7 for it); -1 (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
  * (b >= 16) every candidate is launched twice on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive

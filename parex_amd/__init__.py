@@ -323,6 +323,18 @@ class Server:
         rc = self._check(self._lib.pxd_serve_admit(self._h, graph.n, graph.m, ranges, edges, tag, C.byref(slot)))
         return rc == 0
 
+    def set_walk_variant(self, variant: int):
+        """Kernel choice of the walk and cut contribution, as
+        DeviceManager.setWalkVariant."""
+        self._check(self._lib.pxd_set_walk_variant(self._h, int(variant)))
+
+    def walk_tuning(self):
+        t = np.full(66, -1, np.int32)
+        n = C.c_uint64()
+        self._check(self._lib.pxd_walk_tuning(self._h, t, 66, C.byref(n)))
+        return ({b: int(v) for b, v in enumerate(t[:33]) if v >= 0},
+                {b: int(v) for b, v in enumerate(t[33:]) if v >= 0}, int(n.value))
+
     def enable_edges(self, max_nodes: int, max_edges: int):
         """Workspace for edge-list admissions of graphs up to these sizes."""
         self._check(self._lib.pxd_serve_enable_edges(self._h, int(max_nodes), int(max_edges)))

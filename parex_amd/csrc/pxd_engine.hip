@@ -680,7 +680,10 @@ struct pxd_engine {
             case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
             case WV_WIN2S: launch_walk_win<2, 8, 8192, 1024>(); break;
             case WV_WIN2C: launch_walk_win<2, 8, 20480, 4096, true>(); break;
-            case WV_WIN2R: launch_walk_win<2, 8, 20480, RANGE_R, false, true>(); break;
+            case WV_WIN2R:
+                if (serving) launch_walk_win<2, 8, 20480, 4096>();   // list not in index order
+                else launch_walk_win<2, 8, 20480, RANGE_R, false, true>();
+                break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
     }
@@ -716,8 +719,9 @@ struct pxd_engine {
         if (gs <= 8 && nalist < (1u << 22)) cand.push_back(WV_WIN2S);
         // the walk's clamped-slot form, for long lists of rows that fill their slots
         if (pass == 0 && gs <= 8 && nalist >= (1u << 22)) cand.push_back(WV_WIN2C);
-        // node-range chunks: windows that cover sparse lists too
-        if (gs <= 8) cand.push_back(WV_WIN2R);
+        // node-range chunks: windows that cover sparse lists too (they need the
+        // list in index order, which a serving engine's appended graphs break)
+        if (gs <= 8 && !serving) cand.push_back(WV_WIN2R);
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -764,7 +768,10 @@ struct pxd_engine {
             case WV_WIN2: launch_contrib_win<2, 8, 40960, 4096>(); break;
             case WV_WIN2S: launch_contrib_win<2, 8, 16384, 1024>(); break;
             case WV_WIN2C: launch_contrib_win<2, 8, 40960, 4096>(); break;   // no clamped form
-            case WV_WIN2R: launch_contrib_win<2, 8, 40960, RANGE_R, true>(); break;
+            case WV_WIN2R:
+                if (serving) launch_contrib_win<2, 8, 40960, 4096>();   // list not in index order
+                else launch_contrib_win<2, 8, 40960, RANGE_R, true>();
+                break;
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }
     }

@@ -182,3 +182,25 @@ def test_serve_edge_lists_equal_solo_and_csr_admissions():
     with pytest.raises(px.PxdError, match="edge-admission limits"):
         srv.admit_edges(6000, [0], [1], 5)
     srv.close()
+
+
+@pytest.mark.parametrize("variant", [1, 7, 9, 10, 11])
+def test_serve_pinned_kernel_variants_equal_solo(variant):
+    """A serving engine's node list is a concatenation of the admitted graphs'
+    lists, not one list in index order: every pinned kernel variant still
+    gives each graph its solo result (variant 11, which needs the list in
+    index order, runs as variant 7 there)."""
+    graphs = mix()
+    ref = [solo(g) for g in graphs]
+    srv = px.Server(node_capacity=30000, arc_capacity=400000, graph_slots=16,
+                    params=px.Params(record_node_data=False))
+    srv.set_walk_variant(variant)
+    got = {}
+    # later graphs take lower node ranges freed by earlier ones, so the list runs out of index order
+    drain(srv, list(range(len(graphs)))[::-1], graphs, got)
+    drain(srv, list(range(len(graphs))), graphs, got)
+    for i in range(len(graphs)):
+        ids, k, it = got[i]
+        rids, rk, rit = ref[i]
+        assert k == rk and it == rit and np.array_equal(ids, rids), f"variant {variant}: graph {i} differs from solo"
+    srv.close()
