@@ -4,7 +4,7 @@ A step is one complete decomposition of the workload's graph set (every graph
 run to its final partition).  Inputs are resident in HBM when a step starts:
 `reset` (untimed) restores the engine from the device copy of the input.
 
-Default workload (config 4 of the benchmark plan): 512 independent planted
+Default workload (`--workload c4`): 512 independent planted
 partition graphs, n log-uniform in [1K, 256K], k ~ U{4..64}, intra degree
 ~ U(8, 32), inter degree ~ U(0.5, 3), seed 11; parameters seed 6582,
 sweep threshold 0.3, walk threshold 1e-4.
