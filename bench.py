@@ -4,7 +4,7 @@ A step is one complete decomposition of the workload's graph set (every graph
 run to its final partition).  Inputs are resident in HBM when a step starts:
 `reset` (untimed) restores the engine from the device copy of the input.
 
-Default workload (`--workload c4`): 512 independent planted
+Default workload (`--workload sbm512`): 512 independent planted
 partition graphs, n log-uniform in [1K, 256K], k ~ U{4..64}, intra degree
 ~ U(8, 32), inter degree ~ U(0.5, 3), seed 11; parameters seed 6582,
 sweep threshold 0.3, walk threshold 1e-4.
@@ -39,7 +39,7 @@ def log(*a):
 
 
 def workload(name, graphs_limit=None):
-    if name == "c4":
+    if name == "sbm512":
         specs = px.sbm_specs(512, 1024, 262144, 4, 64, (8.0, 32.0), (0.5, 3.0), 11)
         if graphs_limit:
             specs = specs[:graphs_limit]
@@ -48,13 +48,13 @@ def workload(name, graphs_limit=None):
                             "d_in U(8,32), d_out U(0.5,3), block-diagonal batch",
                 "graphs": len(gs), "seed": 11}
         return gs, desc
-    if name == "c2":
+    if name == "torus1024":
         g = px.torus(1024, 10486, 1)
         return [g], {"workload": "torus 1024x1024 + 0.5% random long-range edges", "graphs": 1, "seed": 1}
-    if name == "c1":
+    if name == "sbm4096":
         g = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
         return [g], {"workload": "SBM n=4096, 16 blocks, p_in=12/255, p_out=1/3840", "graphs": 1, "seed": 6582}
-    if name == "c3":
+    if name == "rmat22":
         g = px.rmat(22, 16, 0.57, 0.19, 0.19, 7)
         return [g], {"workload": "RMAT scale 22, edge factor 16 (.57,.19,.19,.05), symmetrised", "graphs": 1,
                      "seed": 7}
@@ -300,7 +300,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c4")
+    ap.add_argument("--workload", default="sbm512",
+                    choices=["sbm512", "torus1024", "rmat22", "sbm4096", "serve"])
     ap.add_argument("--graphs", type=int, default=None, help="use only the first G graphs of the set")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -315,7 +316,7 @@ def main():
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         return 2
-    if a.workload == "c5":
+    if a.workload == "serve":
         return serve_workload(a)
 
     t = time.perf_counter()
@@ -389,7 +390,7 @@ def main():
                 raise
             tot += time.perf_counter() - t
             ids1, k1 = d1.getFinalPartition()
-            if a.workload == "c4" or len(graphs) > 1:
+            if a.workload == "sbm512" or len(graphs) > 1:
                 if not (np.array_equal(ids1, batch_ids[off:off + g.n]) and k1 == batch_k[i] and it1 == batch_it[i]):
                     same = False
             d1.close()
