@@ -263,6 +263,12 @@ struct pxd_engine {
         HIPC(hipMemcpyAsync(h, d, 4, hipMemcpyDeviceToHost, st));
         return sync();
     }
+    // two device words with one sync
+    int read_u32x2(const uint32_t* d0, uint32_t* h0, const uint32_t* d1, uint32_t* h1) {
+        HIPC(hipMemcpyAsync(h0, d0, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(h1, d1, 4, hipMemcpyDeviceToHost, st));
+        return sync();
+    }
 
     int zero_counter_fields() {
         // counters other than the sticky error word
@@ -613,8 +619,15 @@ struct pxd_engine {
             std::vector<uint32_t> lend(nlarge);
             HIPC(hipMemcpyAsync(lend.data(), d_llen, nlarge * 4, hipMemcpyDeviceToHost, st));
             CHK(sync());
-            std::vector<LargeSeg> ls(nlarge);
-            std::vector<uint32_t> tseg, tidx;
+            // the plan lives in members: the copies below read it after this
+            // function returns, and the next plan is only written after the
+            // next call's classification sync, when these copies are done
+            std::vector<LargeSeg>& ls = h_ls;
+            std::vector<uint32_t>& tseg = h_tseg;
+            std::vector<uint32_t>& tidx = h_tidx;
+            ls.assign(nlarge, LargeSeg{});
+            tseg.clear();
+            tidx.clear();
             uint32_t tiles = 0;
             for (uint32_t i = 0; i < nlarge; ++i) {
                 ls[i].start = lst[i];
@@ -639,10 +652,11 @@ struct pxd_engine {
                 HIPC(hipGetLastError());
                 src ^= 1;
             }
-            CHK(sync());  // host vectors above go out of scope
         }
         return 0;
     }
+    std::vector<LargeSeg> h_ls;
+    std::vector<uint32_t> h_tseg, h_tidx;
 
     // ---------------------------------------------------------------- steps
     template <int GSZ>
@@ -965,8 +979,7 @@ struct pxd_engine {
                                                        d_pclu[1]);
         HIPC(hipGetLastError());
         uint32_t na_new = 0, nc_new = 0;
-        CHK(read_u32(d_tB + na, &na_new));
-        CHK(read_u32(d_kscan + nc, &nc_new));
+        CHK(read_u32x2(d_tB + na, &na_new, d_kscan + nc, &nc_new));
         ClusterTable& T2 = ct[ctc ^ 1];
         if (nc_new) {
             k_clu_compact<<<grid_for(nc, BS), BS, 0, st>>>(nc, na_new, d_ckeep, d_kscan, d_tB, T->cols(), T2.cols());
