@@ -385,3 +385,63 @@ def test_initialize_edges_refuses_an_end_past_n():
         dm.initialize_edges(10, [0, 1, 2, 3], [1, 2, 3, 10])
     dm.initialize_edges(10, [0, 1, 2, 3], [1, 2, 3, 9])
     assert dm.expanderDecomposition() >= 1
+
+
+def _random_graph(rng):
+    kind = rng.integers(0, 6)
+    if kind == 0:
+        n = int(rng.integers(200, 20000))
+        k = int(rng.integers(2, 40))
+        din, dout = rng.uniform(3, 30), rng.uniform(0.2, 3)
+        bs = max(n // k, 2)
+        return px.sbm(n, k, min(din / (bs - 1), 1.0), dout / max(n - bs, 1), int(rng.integers(1 << 30)))
+    if kind == 1:
+        return px.rmat(int(rng.integers(8, 14)), int(rng.integers(2, 17)), 0.57, 0.19, 0.19, int(rng.integers(1 << 30)))
+    if kind == 2:
+        L = int(rng.integers(8, 120))
+        return px.torus(L, int(rng.integers(0, L)), int(rng.integers(1 << 30)))
+    if kind == 3:                                   # random sparse graph, isolated nodes included
+        n = int(rng.integers(50, 5000))
+        m = int(rng.integers(0, 3 * n))
+        fx, tx = rng.integers(0, n, m), rng.integers(0, n, m)
+        keep = fx != tx
+        pairs = np.unique(np.sort(np.stack([fx[keep], tx[keep]]), axis=0), axis=1)
+        return px.from_edges(n, pairs[0], pairs[1])
+    if kind == 4:                                   # star plus path: one long row, long chains
+        n = int(rng.integers(300, 3000))
+        fx = list(range(1, n // 2)) + list(range(n // 2, n - 1))
+        tx = [0] * (n // 2 - 1) + list(range(n // 2 + 1, n))
+        return px.from_edges(n, fx, tx)
+    n = int(rng.integers(4, 40))                    # a clique among isolated nodes
+    iu = np.triu_indices(n, 1)
+    return px.from_edges(n + 10, iu[0], iu[1])
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_graphs_match_oracle(seed):
+    """Random graph families (planted partitions, RMAT, tori, sparse random
+    graphs with isolated nodes, a star joined to a path, a clique), end to end:
+    iterations, cluster ids and their count equal the oracle's."""
+    rng = np.random.default_rng(1000 + seed)
+    g = _random_graph(rng)
+    dm = _engine(g, record_node_data=False)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    assert dm.expanderDecomposition() == ox.run()
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids), f"seed {seed}: n={g.n} m={g.m}"
+    assert ox.errors() == 0
+
+
+def test_tuned_engine_matches_oracle_on_a_large_sbm():
+    """200,000 nodes: lists long enough for the online tuning of both per-arc
+    passes (every length class from 2^16 up); the result equals the oracle's."""
+    g = px.sbm(200000, 40, 16 / 4999, 1.5 / 195000, 29)
+    dm = _engine(g, keep_original=False, record_node_data=False)
+    ox = Oracle(g, init_values=dm.readRandomWalkValuesF64())
+    assert dm.expanderDecomposition() == ox.run()
+    table, ctable, tunings = dm.walkTuning()
+    assert tunings >= 2 and 17 in table
+    ids, k = dm.getFinalPartition()
+    oids, ok = ox.final_partition()
+    assert k == ok and np.array_equal(ids, oids)
