@@ -957,7 +957,7 @@ struct pxd_engine {
         // then the stable regroup): min and max are the first and last keys
         k_stats_ends<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_pnix[0], d_xkey, d_cmaxo, d_cmino);
         k_stats_scale<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK);
-        k_stats_sum<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
+        k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
         k_stats_final<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold,
                                                        d_ckeep, d_cavg);
         HIPC(hipGetLastError());
@@ -1020,7 +1020,7 @@ struct pxd_engine {
         if (nc) {
             ClusterTable& T3 = ct[ctc];
             k_fill_u32<<<grid_for(nc, BS), BS, 0, st>>>(nc, T3.cvol, 0u);
-            k_volume<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
+            k_volume<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
             HIPC(hipGetLastError());
         }
         return 0;

@@ -1454,24 +1454,57 @@ __global__ void k_stats_scale(uint32_t nc, const uint32_t* __restrict__ cs, cons
     }
 }
 
+// Per-cluster sums over the positions.  Clusters are contiguous position
+// ranges, so a workgroup takes SUM_ITEMS * BS consecutive positions at a time:
+// when the first and last are in one cluster, the whole range is, and it is
+// summed over the workgroup and added with one atomic (a cluster of millions
+// of positions would otherwise take one atomic per wave on one address);
+// otherwise each wave adds per cluster run as before.  Integer sums: the
+// order of the additions does not matter.
+constexpr uint32_t SUM_ITEMS = 8;
 __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* __restrict__ pnix,
                                                   const uint32_t* __restrict__ pclu,
                                                   const double* __restrict__ x, const int32_t* __restrict__ cK,
                                                   unsigned long long* __restrict__ csum) {
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
-        const uint32_t k = k0 + threadIdx.x;
-        const bool ok = k < na;
-        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
-        int64_t term = 0;
-        if (ok) term = (int64_t)ldexp((double)(float)x[pnix[k]], cK[c]);
-        const uint32_t c0 = __shfl(c, 0);
-        if (c0 == 0xFFFFFFFFu) {
-            continue;                      // whole wave past the end: nothing to add
-        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
-            for (int off = WAVE / 2; off > 0; off >>= 1) term += __shfl_xor(term, off);
-            if (lane_id() == 0) atomicAdd(&csum[c0], (unsigned long long)term);
-        } else if (ok) {
-            atomicAdd(&csum[c], (unsigned long long)term);
+    __shared__ int64_t lds[BS / WAVE];
+    constexpr uint32_t TR = BS * SUM_ITEMS;
+    for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
+        const uint32_t r1 = min(r0 + TR, na);
+        const uint32_t cf = pclu[r0];
+        if (cf == pclu[r1 - 1]) {
+            const int32_t K = cK[cf];
+            int64_t acc = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
+                const uint32_t k = r0 + i * BS + threadIdx.x;
+                if (k < r1) acc += (int64_t)ldexp((double)(float)x[pnix[k]], K);
+            }
+            acc = group_sum<WAVE>(acc);
+            if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int64_t t = 0;
+                for (uint32_t w = 0; w < BS / WAVE; ++w) t += lds[w];
+                atomicAdd(&csum[cf], (unsigned long long)t);
+            }
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
+            const uint32_t k = r0 + i * BS + threadIdx.x;
+            const bool ok = k < r1;
+            const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+            int64_t term = 0;
+            if (ok) term = (int64_t)ldexp((double)(float)x[pnix[k]], cK[c]);
+            const uint32_t c0 = __shfl(c, 0);
+            if (c0 == 0xFFFFFFFFu) {
+                continue;                      // whole wave past the end: nothing to add
+            } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
+                term = group_sum<WAVE>(term);
+                if (lane_id() == 0) atomicAdd(&csum[c0], (unsigned long long)term);
+            } else if (ok) {
+                atomicAdd(&csum[c], (unsigned long long)term);
+            }
         }
     }
 }
@@ -1817,21 +1850,47 @@ __global__ void __launch_bounds__(BS) k_disable_heavy(const uint32_t* __restrict
     }
 }
 
-__global__ void k_volume(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                         const uint32_t* __restrict__ dint, uint32_t* __restrict__ cvol) {
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < na; k0 += gridDim.x * blockDim.x) {
-        const uint32_t k = k0 + threadIdx.x;
-        const bool ok = k < na;
-        const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
-        uint32_t d = ok ? dint[pnix[k]] : 0u;
-        const uint32_t c0 = __shfl(c, 0);
-        if (c0 == 0xFFFFFFFFu) {
-            continue;                      // whole wave past the end: nothing to add
-        } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
-            d = group_sum<WAVE>(d);
-            if (lane_id() == 0) atomicAdd(&cvol[c0], d);
-        } else if (ok) {
-            atomicAdd(&cvol[c], d);
+// cvol[c] = sum of dInt over the cluster's positions (the k_stats_sum layout)
+__global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __restrict__ pnix,
+                                               const uint32_t* __restrict__ pclu,
+                                               const uint32_t* __restrict__ dint, uint32_t* __restrict__ cvol) {
+    __shared__ uint32_t lds[BS / WAVE];
+    constexpr uint32_t TR = BS * SUM_ITEMS;
+    for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
+        const uint32_t r1 = min(r0 + TR, na);
+        const uint32_t cf = pclu[r0];
+        if (cf == pclu[r1 - 1]) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
+                const uint32_t k = r0 + i * BS + threadIdx.x;
+                if (k < r1) acc += dint[pnix[k]];
+            }
+            acc = group_sum<WAVE>(acc);
+            if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t t = 0;
+                for (uint32_t w = 0; w < BS / WAVE; ++w) t += lds[w];
+                atomicAdd(&cvol[cf], t);
+            }
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
+            const uint32_t k = r0 + i * BS + threadIdx.x;
+            const bool ok = k < r1;
+            const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+            uint32_t d = ok ? dint[pnix[k]] : 0u;
+            const uint32_t c0 = __shfl(c, 0);
+            if (c0 == 0xFFFFFFFFu) {
+                continue;                      // whole wave past the end: nothing to add
+            } else if (__all(c == c0 || c == 0xFFFFFFFFu)) {
+                d = group_sum<WAVE>(d);
+                if (lane_id() == 0) atomicAdd(&cvol[c0], d);
+            } else if (ok) {
+                atomicAdd(&cvol[c], d);
+            }
         }
     }
 }
