@@ -1133,13 +1133,33 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
             nd[k] = o;
         }
     }
-    // one atomic per wave when the whole wave ends in the same cluster
+    // the last runs: one atomic per workgroup when all of them are one cluster
+    // (a giant cluster would otherwise take one atomic per wave on one
+    // address), else one per wave when the wave's are, else one per thread
+    __shared__ unsigned long long s_min[BS / WAVE];
+    __shared__ uint32_t s_c[BS / WAVE];
     const uint32_t c0 = __shfl(runc, 0);
-    if (__all(runc == c0) && c0 != 0xFFFFFFFFu) {
+    const bool wu = __all(runc == c0) && c0 != 0xFFFFFFFFu;
+    if (wu) {
         for (int offs = WAVE / 2; offs > 0; offs >>= 1) {
             const unsigned long long o = __shfl_xor(runmin, offs);
             runmin = o < runmin ? o : runmin;
         }
+    }
+    if (lane_id() == 0) {
+        s_c[t / WAVE] = wu ? c0 : 0xFFFFFFFFu;
+        s_min[t / WAVE] = runmin;
+    }
+    __syncthreads();
+    bool bu = s_c[0] != 0xFFFFFFFFu;
+    for (uint32_t w = 1; w < BS / WAVE; ++w) bu = bu && s_c[w] == s_c[0];
+    if (bu) {
+        if (t == 0) {
+            unsigned long long m = s_min[0];
+            for (uint32_t w = 1; w < BS / WAVE; ++w) m = s_min[w] < m ? s_min[w] : m;
+            atomicMin(&cbest[s_c[0]], m);
+        }
+    } else if (wu) {
         if (lane_id() == 0) atomicMin(&cbest[c0], runmin);
     } else if (runc != 0xFFFFFFFFu) {
         atomicMin(&cbest[runc], runmin);
