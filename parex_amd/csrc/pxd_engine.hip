@@ -1035,8 +1035,20 @@ struct pxd_engine {
     // their part's giant tree.  The regroup pass (k_rid) leaves every tree flat.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
+        // the LDS window when the walk's choice for this list length is a window
+        // kernel (neighbours near in index order) and the list fills the chip;
+        // at any length when a window variant is pinned
+        const int wv = walk_force >= 0 ? walk_force
+                                       : (nalist >= TUNE_MIN ? tuned[0][31 - __builtin_clz(nalist)] : -1);
+        const bool win = wv >= WV_WIN1 && (walk_force >= 0 || nl >= (1u << 20));
         for (uint32_t r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
-            k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r);
+            if (win) {
+                const uint32_t grid = (std::min<uint32_t>(cdiv(nl, 4096), 4096u) + NXCD - 1) / NXCD * NXCD;
+                k_cc_sample_win<20480, 4096><<<grid, WBS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part,
+                                                                   d_root, r, N);
+            } else {
+                k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r);
+            }
             k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
         }
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);

@@ -1261,6 +1261,61 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
     }
 }
 
+// k_cc_sample with an LDS window of the parts and pointers (the layout of
+// k_walk_win: chunks of CH entries of the member list, WIN values around a
+// chunk staged when its nodes span less than WIN/2).  The staged pointers are
+// those after the preceding jump launch; a stale pointer is still a smaller
+// node of the same component, so links stay valid.  The slot lowered is
+// checked against its current value, as in k_cc_sample.
+template <int WIN, int CH>
+__global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ row,
+                                                       const uint32_t* __restrict__ dint,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       const uint32_t* __restrict__ part,
+                                                       uint32_t* __restrict__ root, uint32_t r, uint32_t N) {
+    __shared__ uint32_t sp_[WIN], sr_[WIN];
+    const uint32_t nch = (nl + CH - 1) / CH;
+    const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
+    const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
+    const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
+    for (uint32_t c = cb + lb; c < ce; c += nb) {
+        const uint32_t c0 = c * CH, c1 = min(c0 + CH, nl);
+        const uint32_t uf = list[c0], ul = list[c1 - 1];
+        const bool use = ul >= uf && ul - uf < (uint32_t)WIN / 2;
+        uint32_t wb = 0;
+        if (use) {
+            const uint32_t mid = uf / 2 + ul / 2;
+            wb = mid > (uint32_t)WIN / 2 ? mid - (uint32_t)WIN / 2 : 0u;
+            if (N <= (uint32_t)WIN) wb = 0;
+            else if (wb > N - (uint32_t)WIN) wb = N - (uint32_t)WIN;
+            for (uint32_t k = threadIdx.x; k < (uint32_t)WIN; k += WBS) {
+                if (wb + k < N) {
+                    sp_[k] = part[wb + k];
+                    sr_[k] = ld_dev(&root[wb + k]);
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = c0 + threadIdx.x; i < c1; i += WBS) {
+            const uint32_t u = list[i];
+            if (r >= dint[u]) continue;
+            const uint32_t v = nbr[row[u] + r];
+            const uint32_t ou = u - wb, ov = v - wb;
+            const bool iu = use && ou < (uint32_t)WIN, iv = use && ov < (uint32_t)WIN;
+            const uint32_t pu = iu ? sp_[ou] : part[u];
+            const uint32_t pv = iv ? sp_[ov] : part[v];
+            if (pv != pu) continue;
+            const uint32_t a = iu ? sr_[ou] : root[u];
+            const uint32_t b = iv ? sr_[ov] : root[v];
+            if (a == b) continue;
+            const uint32_t hi = max(a, b), lo = min(a, b);
+            if (root[hi] > lo) atomicMin(&root[hi], lo);
+        }
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
     for (;;) {                        // strictly decreasing: ends at a root
         const uint32_t p = ld_dev(&root[r]);

@@ -92,9 +92,9 @@ WALK_VARIANT_GRAPHS = {
 @pytest.mark.parametrize("variant", range(12))
 def test_walk_variants_lockstep(variant):
     """Every kernel variant of the walk and the cut contribution -- 2..64
-    lanes per row, the LDS window with 1, 2, 4 lanes per row -- gives the
-    oracle's bits (walk values, NodeData prefixes, sweep cuts, labels),
-    through cuts and arc disabling."""
+    lanes per row, the LDS-window kernels -- gives the oracle's bits (walk
+    values, NodeData prefixes, sweep cuts, labels), through cuts, CC sampling
+    (the window form when a window variant is pinned) and arc disabling."""
     for name, make in WALK_VARIANT_GRAPHS.items():
         g = make()
         dm = _engine(g)
