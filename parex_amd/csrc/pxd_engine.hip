@@ -121,7 +121,7 @@ struct pxd_engine {
     double* d_x[2] = {nullptr, nullptr};
     double* d_y[2] = {nullptr, nullptr};
     uint32_t* d_xkey = nullptr;
-    int32_t* d_ediff = nullptr;
+    uint2* d_ediff = nullptr;   // per node: cut contribution and dInt of the current sweep
     uint32_t *d_ped = nullptr, *d_pvol = nullptr;   // cut contribution and dInt in position order (sweep)
     uint32_t *d_root = nullptr, *d_part = nullptr, *d_stamp = nullptr;
     uint8_t* d_alive = nullptr;
@@ -897,7 +897,7 @@ struct pxd_engine {
         }
         k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, nullptr, nullptr, nullptr, d_cbest);
         const uint32_t nb = cdiv(na, BS * SCAN_ITEMS);
-        k_sweep_a<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, d_ediff, d_dint, d_ped, d_pvol, d_tail_ed,
+        k_sweep_a<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, d_ediff, d_ped, d_pvol, d_tail_ed,
                                      d_tail_vol, d_hasst);
         HIPC(hipGetLastError());
         k_sweep_b<<<1, 1024, 0, st>>>(nb, d_tail_ed, d_tail_vol, d_hasst, d_carry_ed, d_carry_vol);

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
 
 // ------------------------------------------------------------------ cut contribution (K4)
 
-// ediff[u] = sum over valid arcs of (neighbour sorts before u ? -1 : +1), where
+// edv[u].x = sum over valid arcs of (neighbour sorts before u ? -1 : +1), where
 // "before" compares walk keys and breaks ties by node index
 // (cuda/devSweepCut.h:20-112).
 template <int GS>
@@ -254,14 +254,14 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ xkey,
-                                                int32_t* __restrict__ ediff, uint32_t heavy) {
+                                                uint2* __restrict__ edv, uint32_t heavy) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
         if (d > heavy) {              // k_contrib_heavy adds the chunks' counts
-            if (lane == 0) ediff[u] = 0;
+            if (lane == 0) edv[u] = make_uint2(0u, d);
             continue;
         }
         const uint32_t mk = xkey[u];
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(BS) k_contrib(uint32_t na, const uint32_t* __r
             }
         }
         c = group_sum<GS>(c);
-        if (lane == 0) ediff[u] = c;
+        if (lane == 0) edv[u] = make_uint2((uint32_t)c, d);
     }
 }
 
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t
                                                      const uint32_t* __restrict__ dint,
                                                      const uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ xkey,
-                                                     int32_t* __restrict__ ediff, uint32_t heavy, uint32_t N,
+                                                     uint2* __restrict__ edv, uint32_t heavy, uint32_t N,
                                                      const uint32_t* __restrict__ coff = nullptr) {
     static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of key quads, lane groups within a wave");
     __shared__ uint32_t sk[WIN];
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t
                 const int32_t got = __shfl(cnt, (lane % NPR) * GS);
                 if (lane / NPR == (uint32_t)r) res = got;
             }
-            if (ok) ediff[u] = light ? res : 0;   // k_contrib_heavy adds the long rows' chunks
+            if (ok) edv[u] = make_uint2(light ? (uint32_t)res : 0u, dd);   // k_contrib_heavy adds long rows' chunks
         }
         __syncthreads();
     }
@@ -640,7 +640,7 @@ __global__ void k_walk_heavy_finish(const uint32_t* __restrict__ nchunks, const 
     if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
 }
 
-// cut contribution of one chunk of a heavy row, added to ediff[u] (exact integers)
+// cut contribution of one chunk of a heavy row, added to edv[u].x (exact integers)
 __global__ void __launch_bounds__(BS) k_contrib_heavy(const uint32_t* __restrict__ nchunks,
                                                       const uint32_t* __restrict__ hrow,
                                                       const uint32_t* __restrict__ hoff,
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(BS) k_contrib_heavy(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ dint,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint32_t* __restrict__ xkey,
-                                                      int32_t* __restrict__ ediff) {
+                                                      uint2* __restrict__ edv) {
     __shared__ int32_t lds[BS / WAVE];
     const uint32_t H = *nchunks;
     for (uint32_t b = blockIdx.x; b < H; b += gridDim.x) {
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(BS) k_contrib_heavy(const uint32_t* __restrict
         if (threadIdx.x == 0) {
             int32_t t = 0;
             for (uint32_t w = 0; w < BS / WAVE; ++w) t += lds[w];
-            atomicAdd(&ediff[u], t);
+            atomicAdd(&edv[u].x, (uint32_t)t);   // two's complement: the signed sum
         }
         __syncthreads();
     }
@@ -976,8 +976,7 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint
 __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __restrict__ pnix,
                                                 const uint32_t* __restrict__ pclu,
                                                 const uint32_t* __restrict__ cs,
-                                                const int32_t* __restrict__ ediff,
-                                                const uint32_t* __restrict__ dint,
+                                                const uint2* __restrict__ edv,
                                                 uint32_t* __restrict__ ped, uint32_t* __restrict__ pvol,
                                                 uint32_t* __restrict__ tail_ed, uint32_t* __restrict__ tail_vol,
                                                 uint32_t* __restrict__ has_start) {
@@ -995,7 +994,8 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
         if (cs[c] == k) st = 1u;
         // the node's values gathered once, kept in position order for k_sweep_c
         const uint32_t u = pnix[k];
-        const uint32_t e = (uint32_t)ediff[u], d = dint[u];
+        const uint2 ev = edv[u];      // the cut contribution and dInt, one gather
+        const uint32_t e = ev.x, d = ev.y;
         ped[k] = e;
         pvol[k] = d;
         if (c == lastc) {

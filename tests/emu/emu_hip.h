@@ -30,6 +30,8 @@
 struct emu_dim3 { uint32_t x = 0, y = 1, z = 1; };
 struct double2 { double x, y; };
 struct uint4 { uint32_t x, y, z, w; };
+struct uint2 { uint32_t x, y; };
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
 extern thread_local emu_dim3 threadIdx, blockIdx, blockDim, gridDim;
 
 namespace emu {
