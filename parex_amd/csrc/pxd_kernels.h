@@ -1324,6 +1324,17 @@ __device__ __forceinline__ uint32_t cc_find(uint32_t* root, uint32_t r) {
     }
 }
 
+// find with plain (cached) loads, for launches in which no root moves: the
+// only pointer writes are a thread's own slot set to that slot's root, so
+// an old value read from a cache is still on the same path to the same root
+__device__ __forceinline__ uint32_t cc_find_stable(const uint32_t* __restrict__ root, uint32_t r) {
+    for (;;) {
+        const uint32_t p = root[r];
+        if (p == r) return r;
+        r = p;
+    }
+}
+
 // Giant root of every part after sampling: the root of the part's middle
 // member (by position).  Members already in that tree skip the exact phase.
 __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
@@ -1334,8 +1345,8 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
         if (!(f & CF_CC)) continue;
         const uint32_t b = cs[c], len = cs[c + 1] - b;
         const uint32_t o = (f & CF_CUT) ? min(coff[c], len) : len;
-        giant[c << 1] = o ? cc_find(root, pnix[b + o / 2]) : INVALID;
-        giant[(c << 1) | 1] = (len > o) ? cc_find(root, pnix[b + o + (len - o) / 2]) : INVALID;
+        giant[c << 1] = o ? cc_find_stable(root, pnix[b + o / 2]) : INVALID;
+        giant[(c << 1) | 1] = (len > o) ? cc_find_stable(root, pnix[b + o + (len - o) / 2]) : INVALID;
     }
 }
 
@@ -1345,7 +1356,7 @@ __global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, co
                               uint32_t* __restrict__ f) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
-        f[i] = cc_find(root, u) != giant[part[u]] ? 1u : 0u;
+        f[i] = cc_find_stable(root, u) != giant[part[u]] ? 1u : 0u;
     }
 }
 
@@ -1453,7 +1464,7 @@ __global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint
         const uint32_t c = pclu[k];
         if (cflag[c] & CF_CC) {
             const uint32_t u = pnix[k];
-            const uint32_t r = cc_find(root, u);
+            const uint32_t r = cc_find_stable(root, u);
             st_dev(&root[u], r);      // own slot; readers see the old or the new pointer, both on u's path
             key[k] = r;
         } else {
