@@ -693,7 +693,7 @@ struct pxd_engine {
             case WV_WIN1: launch_walk_win<1, 8, 20480, 4096>(); break;
             case WV_WIN2: launch_walk_win<2, 8, 20480, 4096>(); break;
             case WV_WIN2S: launch_walk_win<2, 8, 8192, 1024>(); break;
-            case WV_WIN2C: launch_walk_win<2, 8, 20480, 4096, true>(); break;
+            case WV_WIN2C: launch_walk_win<2, 6, 20480, 4096, true>(); break;
             case WV_WIN2R:
                 if (serving) launch_walk_win<2, 8, 20480, 4096>();   // list not in index order
                 else launch_walk_win<2, 8, 20480, RANGE_R, false, true>();
