@@ -295,13 +295,70 @@ def cpu_baseline(graphs, budget_s):
             "sample": f"first {k} graphs ({e} edges) through the oracle, {dt:.2f} s on 1 core"}
 
 
+def hierarchy_workload(a, k=16):
+    """Expander hierarchy (every level decomposed by the engine, contracted on
+    the host; lib/algorithms/expander_hierarchy.h:107-140) and a k-way
+    normalized cut (lib/algorithms/normalized_cut.h:108-144) of one connected
+    graph.  value = input edges / (build + cut) seconds."""
+    g = px.torus(1024, 10486, 1)
+    cfg = {"workload": f"expander hierarchy + {k}-way normalized cut of torus 1024x1024 + 0.5% random "
+                       "long-range edges", "graphs": 1, "seed": 1, "nodes": g.n, "edges": g.m,
+           "params": {"seed": 6582, "target_sparsity": 0.2, "rw_threshold": 1e-5, "k": k}}
+
+    def once():
+        h = px.ExpanderHierarchy(g)
+        t0 = time.perf_counter()
+        h.build()
+        t1 = time.perf_counter()
+        labels, val = px.NormalizedCut(h).compute(k)
+        t2 = time.perf_counter()
+        return h, t1 - t0, t2 - t1, val
+
+    for w in range(a.warmup):
+        _, b, c, _ = once()
+        log(f"warmup {w}: build {b * 1e3:.1f} ms, cut {c * 1e3:.1f} ms")
+    bs, cs = [], []
+    for s in range(a.steps):
+        h, b, c, val = once()
+        bs.append(b)
+        cs.append(c)
+        log(f"step {s}: build {b * 1e3:.1f} ms, cut {c * 1e3:.1f} ms, objective {val:.6f}")
+    levels = [h.lvl(i).graph.n for i in range(h.height())]
+    sec = float(np.mean(bs) + np.mean(cs))
+    cpu = None
+    ref = os.path.join(ROOT, "oracle", "_ref", "parex_cpu_ref")
+    if not a.no_cpu and os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            p = os.path.join(td, "g.pxg")
+            px.write_pxg(p, g)
+            cmd = [ref, "hierarchy", p, "6582", str(k)]
+            if shutil.which("taskset"):
+                cmd = ["taskset", "-c", "0", *cmd]
+            out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600).stdout
+            r = json.loads([line for line in out.splitlines() if line.startswith("{")][-1])
+            cpu = {"value": g.m / (r["build_s"] + r["nc_s"]), "unit": "edges/s", "cores": 1, "kind": "reference",
+                   "sample": f"the whole workload: ExpanderHierarchy::build<false> (CPU decompositions, "
+                             f"{len(r['levels'])} levels) {r['build_s']:.2f} s + NormalizedCut::compute({k}) "
+                             f"{r['nc_s']:.2f} s on 1 core"}
+    line = {
+        "metric": "input edges/s through the expander hierarchy build and a normalized cut",
+        "value": g.m / sec, "unit": "edges/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic", "config": cfg,
+        "build_ms": float(np.mean(bs) * 1e3), "cut_ms": float(np.mean(cs) * 1e3),
+        "levels": levels, "objective": val, "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sbm512",
-                    choices=["sbm512", "torus1024", "rmat22", "sbm4096", "serve"])
+                    choices=["sbm512", "torus1024", "rmat22", "sbm4096", "serve", "hierarchy"])
     ap.add_argument("--graphs", type=int, default=None, help="use only the first G graphs of the set")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -318,6 +375,8 @@ def main():
         return 2
     if a.workload == "serve":
         return serve_workload(a)
+    if a.workload == "hierarchy":
+        return hierarchy_workload(a)
 
     t = time.perf_counter()
     graphs, cfg = workload(a.workload, a.graphs)

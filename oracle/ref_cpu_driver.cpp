@@ -12,7 +12,14 @@
 //       lib/core/partition.h:309-362 (Partition::sweepCut on cluster 0)
 //   parex_cpu_ref verify <g.pxg> <ids.i32> <numClusters>
 //       lib/core/partition.h:233-295 (splitByIndices<false,false>, main.cpp:54-60)
+//   parex_cpu_ref hierarchy <g.pxg> <seed> <k>
+//       lib/algorithms/expander_hierarchy.h:107-140 (build<false>, CPU
+//       decompositions) then lib/algorithms/normalized_cut.h:108-144 (compute(k))
+//       and lib/core/partition.h:418-427; prints every level, the size, volume
+//       and internal volume of each of the k parts and the objective as one
+//       JSON line.  The graph must be connected.
 #include "algorithms/expander_decomposition.h"
+#include "algorithms/normalized_cut.h"
 
 #include <chrono>
 #include <cstdio>
@@ -103,6 +110,59 @@ int main(int argc, char** argv) {
         std::vector<NodeIx> mod;
         part.splitByIndices<false, false>(0, ids, k, mod);
         printf("{\"numClusters\":%u,\"cutEdges\":%u}\n", part.numClusters(), part.getNumCutEdges());
+        return 0;
+    }
+    if (mode == "hierarchy" && argc >= 5) {
+        Graph G = load_pxg(argv[2]);
+        random_source.seed((unsigned)std::strtoul(argv[3], nullptr, 10));
+        NodeIx k = (NodeIx)std::strtoul(argv[4], nullptr, 10);
+        ExpanderHierarchy eh{&G};
+        auto t0 = std::chrono::steady_clock::now();
+        eh.build();
+        auto t1 = std::chrono::steady_clock::now();
+        auto arr = [](const char* name, const auto& v, bool comma) {
+            printf("\"%s\":[", name);
+            for (size_t i = 0; i < v.size(); ++i) printf(i ? ",%u" : "%u", (unsigned)v[i]);
+            printf(comma ? "]," : "]");
+        };
+        printf("{\"levels\":[");
+        for (NodeIx l = 0; l < eh.height(); ++l) {
+            const auto& L = eh.lvl(l);
+            std::vector<NodeIx> off{0}, members;
+            for (const Cluster& c : L.partition) {
+                for (const ClusterVertex& cv : c) members.push_back(cv.nix);
+                off.push_back((NodeIx)members.size());
+            }
+            printf(l ? ",{" : "{");
+            arr("member_off", off, true);
+            arr("members", members, true);
+            arr("ranges", L.graph.ranges, true);
+            arr("edges", L.graph.edges, true);
+            arr("volumes", L.volumes, true);
+            arr("cut_sizes", L.cutSizes, true);
+            arr("parents", L.parents, false);
+            printf("}");
+        }
+        NormalizedCut nc{&eh};
+        auto t2 = std::chrono::steady_clock::now();
+        Partition kp = nc.compute(k);
+        auto t3 = std::chrono::steady_clock::now();
+        // splitByIndices<false,false> keeps the node order, so a part's member
+        // range is positional; its size, volume and internal volume are those
+        // of label i (partition.h:239-270), cluster i holding label i
+        std::vector<NodeIx> size, vol, inner;
+        for (const Cluster& c : kp) {
+            size.push_back(c.size());
+            vol.push_back(c.volume);
+            inner.push_back(c.internalVolume);
+        }
+        printf("],");
+        arr("nc_size", size, true);
+        arr("nc_volume", vol, true);
+        arr("nc_internal", inner, true);
+        printf("\"build_s\":%.6f,\"nc_s\":%.6f,", std::chrono::duration<double>(t1 - t0).count(),
+               std::chrono::duration<double>(t3 - t2).count());
+        printf("\"nc_value\":%.9g}\n", compute_normalized_cut(kp));
         return 0;
     }
     fprintf(stderr, "bad arguments\n");

@@ -601,3 +601,119 @@ class DeviceManager:
     get_num_cut_edges = getNumCutEdges
     get_num_clusters = getNumClusters
     update_labels = updateLabels
+
+
+# --------------------------------------------------------------------------- hierarchy
+
+def pxh() -> C.CDLL:
+    lib = _load("libpxh.so")
+    if not getattr(lib, "_typed", False):
+        P = C.c_void_p
+        lib.pxh_last_error.restype = C.c_char_p
+        lib.pxh_create.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_void_p)]
+        lib.pxh_destroy.argtypes = [P]
+        lib.pxh_build.argtypes = [P, C.POINTER(_Params), C.c_float, C.c_float, C.c_uint32]
+        lib.pxh_add_level.argtypes = [P, C.c_uint32, _u32p, _u32p]
+        lib.pxh_height.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.pxh_level_size.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint32)]
+        lib.pxh_level.argtypes = [P, C.c_uint32] + [_u32p] * 7
+        lib.pxh_normalized_cut.argtypes = [P, C.c_uint32, _i32p, C.POINTER(C.c_float)]
+        lib._typed = True
+    return lib
+
+
+# C-ABI entry points declared in include/parex_amd/pxh.h (checked by the tests)
+PXH_SYMBOLS = [
+    "pxh_last_error", "pxh_create", "pxh_destroy", "pxh_build", "pxh_add_level", "pxh_height",
+    "pxh_level_size", "pxh_level", "pxh_normalized_cut",
+]
+
+
+@dataclass
+class Level:
+    """One level of the hierarchy (lib/algorithms/expander_hierarchy.h:17-23):
+    the contracted multigraph, the subtree volumes and cut sizes of its nodes,
+    the cluster of each node below, and the partition of the graph below."""
+
+    graph: Graph
+    volumes: np.ndarray
+    cut_sizes: np.ndarray
+    parents: np.ndarray
+    member_off: np.ndarray
+    members: np.ndarray
+
+
+class ExpanderHierarchy:
+    """Mirror of ``ExpanderHierarchy`` (lib/algorithms/expander_hierarchy.h:13-160)
+    whose decompositions run on the HIP engine (libpxh.so over libpxd.so)."""
+
+    def __init__(self, graph: Graph):
+        self._lib = pxh()
+        self._graph = graph
+        h = C.c_void_p()
+        self._check(self._lib.pxh_create(graph.n, graph.m, np.ascontiguousarray(graph.ranges, np.uint32),
+                                         np.ascontiguousarray(graph.edges, np.uint32), C.byref(h)))
+        self._h = h
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise PxdError(rc, self._lib.pxh_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pxh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # expander_hierarchy.h:107-140 (Config{} defaults: targetSparsity 0.2, randomWalkThreshold 1e-5)
+    def build(self, params: Params | None = None, target_sparsity: float = 0.2, rw_threshold: float = 1e-5,
+              max_retries: int = 64):
+        p = params or Params()
+        cp = _Params(p.seed, p.sc_threshold, p.rw_threshold, p.max_iter, 0, 0, p.device)
+        self._check(self._lib.pxh_build(self._h, C.byref(cp), target_sparsity, rw_threshold, max_retries))
+
+    # expander_hierarchy.h:29-88 for a given partition of the top graph
+    def add_level(self, member_off: np.ndarray, members: np.ndarray):
+        off = np.ascontiguousarray(member_off, np.uint32)
+        self._check(self._lib.pxh_add_level(self._h, len(off) - 1, off, np.ascontiguousarray(members, np.uint32)))
+
+    def getGraph(self) -> Graph:
+        return self._graph
+
+    def height(self) -> int:
+        v = C.c_uint32()
+        self._check(self._lib.pxh_height(self._h, C.byref(v)))
+        return int(v.value)
+
+    def lvl(self, level: int) -> Level:
+        k, arcs, nb = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        self._check(self._lib.pxh_level_size(self._h, level, C.byref(k), C.byref(arcs), C.byref(nb)))
+        k, arcs, nb = int(k.value), int(arcs.value), int(nb.value)
+        a = [np.zeros(s, np.uint32) for s in (k + 1, max(arcs, 1), k, k, nb, k + 1, nb)]
+        self._check(self._lib.pxh_level(self._h, level, *a))
+        ranges, edges, vol, cut, parents, off, members = a
+        return Level(Graph(k, arcs // 2, ranges, edges[:arcs]), vol, cut, parents, off, members)
+
+    get_graph = getGraph
+
+
+class NormalizedCut:
+    """Mirror of ``NormalizedCut`` (lib/algorithms/normalized_cut.h:12-145)."""
+
+    def __init__(self, hierarchy: ExpanderHierarchy):
+        self.hierarchy = hierarchy
+
+    # normalized_cut.h:108-144: part of each input node, and the objective
+    # (lib/core/partition.h:418-427) of that partition
+    def compute(self, k: int):
+        h = self.hierarchy
+        labels = np.zeros(h.getGraph().n, np.int32)
+        val = C.c_float()
+        h._check(h._lib.pxh_normalized_cut(h._h, k, labels, C.byref(val)))
+        return labels, float(val.value)

@@ -3,6 +3,7 @@ cross-compiles gfx950 code without a GPU).
 
   parex_amd/libpxgen.so   host graph ingest (gcc, OpenMP)
   parex_amd/libpxd.so     HIP engine + C ABI (hipcc --offload-arch=gfx950)
+  parex_amd/libpxh.so     expander hierarchy + normalized cut (g++, links libpxd)
   parex_amd/bin/parex     CLI drop-in (g++, links both)
 
 The oracle (test infrastructure) is built by oracle/Makefile.
@@ -60,6 +61,13 @@ def build_cli():
           "-Wl,-rpath,$ORIGIN/.."])
 
 
+def build_hierarchy():
+    """libpxh.so: the hierarchy and normalized cut over the engine (pxh.h)."""
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-ffp-contract=off", f"-I{INC}",
+          os.path.join(CSRC, "pxh.cpp"), "-o", os.path.join(PKG, "libpxh.so"), f"-L{PKG}", "-lpxd",
+          "-Wl,-rpath,$ORIGIN"])
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "all"])
 
@@ -68,6 +76,7 @@ def build(oracle: bool = True):
     build_pxgen()
     build_engine()
     build_cli()
+    build_hierarchy()
     if oracle:
         build_oracle()
 
