@@ -3,7 +3,7 @@ cross-compiles gfx950 code without a GPU).
 
   parex_amd/libpxgen.so   host graph ingest (gcc, OpenMP)
   parex_amd/libpxd.so     HIP engine + C ABI (hipcc --offload-arch=gfx950)
-  parex_amd/libpxh.so     expander hierarchy + normalized cut (g++, links libpxd)
+  parex_amd/libpxh.so     expander hierarchy + normalized cut (g++, OpenMP, links libpxd)
   parex_amd/bin/parex     CLI drop-in (g++, links both)
 
 The oracle (test infrastructure) is built by oracle/Makefile.
@@ -63,7 +63,7 @@ def build_cli():
 
 def build_hierarchy():
     """libpxh.so: the hierarchy and normalized cut over the engine (pxh.h)."""
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-ffp-contract=off", f"-I{INC}",
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-ffp-contract=off", "-fopenmp", f"-I{INC}",
           os.path.join(CSRC, "pxh.cpp"), "-o", os.path.join(PKG, "libpxh.so"), f"-L{PKG}", "-lpxd",
           "-Wl,-rpath,$ORIGIN"])
 

@@ -77,16 +77,23 @@ int contract(pxh_hierarchy* h, uint32_t k, const uint32_t* member_off, const uin
     L.cut_sizes.assign(k, 0);
     L.ranges.assign(k + 1, 0);
     const bool first = h->levels.empty();
+    const uint32_t* lower_vol = first ? nullptr : h->levels.back().volumes.data();
+    // clusters are independent: each thread owns whole clusters
+    #pragma omp parallel for schedule(dynamic, 64)
     for (uint32_t c = 0; c < k; ++c) {
+        uint32_t vol = 0, cut = 0;
         for (uint32_t i = member_off[c]; i < member_off[c + 1]; ++i) {
             uint32_t v = members[i];
-            L.volumes[c] += first ? row[v + 1] - row[v] : h->levels.back().volumes[v];
+            vol += lower_vol ? lower_vol[v] : row[v + 1] - row[v];
             for (uint32_t a = row[v]; a < row[v + 1]; ++a)
-                if (L.parents[nbr[a]] != c) L.cut_sizes[c]++;
+                if (L.parents[nbr[a]] != c) cut++;
         }
-        L.ranges[c + 1] = L.ranges[c] + L.cut_sizes[c];
+        L.volumes[c] = vol;
+        L.cut_sizes[c] = cut;
     }
+    for (uint32_t c = 0; c < k; ++c) L.ranges[c + 1] = L.ranges[c] + L.cut_sizes[c];
     L.edges.resize(L.ranges[k]);
+    #pragma omp parallel for schedule(dynamic, 64)
     for (uint32_t c = 0; c < k; ++c) {
         uint32_t o = L.ranges[c];
         for (uint32_t i = member_off[c]; i < member_off[c + 1]; ++i) {
@@ -140,7 +147,14 @@ class NormalizedCutRun {
     const pxh_hierarchy* h;
     std::vector<std::vector<uint32_t>> vol, cut;
     std::vector<int32_t> labels;
-    std::set<TreeNode> roots;
+    std::set<TreeNode> roots;                   // iterated in (level, nix) order
+    std::vector<std::vector<uint8_t>> is_root;  // the same set, for O(1) membership
+    bool write_labels = false;
+
+    void add_root(const TreeNode& tn) {
+        roots.insert(tn);
+        is_root[tn.level][tn.nix] = 1;
+    }
 
     // normalized_cut.h:43-51
     float edge_penalty(const TreeNode& tn, uint32_t cluster_cut, uint32_t cluster_vol) const {
@@ -151,18 +165,23 @@ class NormalizedCutRun {
         return as_frac(cut_a) / as_frac(vol_a) + as_frac(cut_b) / as_frac(vol_b);
     }
 
-    // normalized_cut.h:56-82: children in member order; ties keep the first
+    // normalized_cut.h:56-82: children in member order; ties keep the first.
+    // A level-0 cluster has no tree-node children: its call only labels the
+    // input nodes.  Every round labels every input node again (each lies under
+    // exactly one nearest root), so only the last round's labels are kept and
+    // earlier rounds skip those calls.
     Candidate candidate(const TreeNode& tn, int32_t label, uint32_t cluster_cut, uint32_t cluster_vol) {
         Candidate best{TreeNode{0, 0}, 100000.0f};
         const Level& L = h->levels[tn.level];
+        if (tn.level == 0) {
+            if (write_labels)
+                for (uint32_t i = L.member_off[tn.nix]; i < L.member_off[tn.nix + 1]; ++i) labels[L.members[i]] = label;
+            return best;
+        }
         for (uint32_t i = L.member_off[tn.nix]; i < L.member_off[tn.nix + 1]; ++i) {
             const uint32_t v = L.members[i];
-            if (tn.level == 0) {
-                labels[v] = label;
-                continue;
-            }
             const TreeNode child{tn.level - 1, v};
-            if (roots.count(child)) continue;
+            if (is_root[child.level][child.nix]) continue;
             if (vol[child.level][child.nix] < cluster_vol) {
                 float pen = edge_penalty(child, cluster_cut, cluster_vol);
                 if (pen < best.penalty) best = {child, pen};
@@ -170,6 +189,36 @@ class NormalizedCutRun {
             Candidate cd = candidate(child, label, cluster_cut, cluster_vol);
             if (cd.penalty < best.penalty) best = cd;
         }
+        return best;
+    }
+
+    // candidate() of a root with its children's subtrees evaluated in
+    // parallel.  The serial loop keeps the first candidate of least penalty in
+    // child order (strict <), a fold that the in-order reduction of the
+    // per-child results reproduces; the final round's label writes are
+    // disjoint between subtrees.
+    Candidate root_candidate(const TreeNode& tn, int32_t label, uint32_t cluster_cut, uint32_t cluster_vol) {
+        if (tn.level == 0) return candidate(tn, label, cluster_cut, cluster_vol);
+        const Level& L = h->levels[tn.level];
+        const uint32_t b = L.member_off[tn.nix], e = L.member_off[tn.nix + 1];
+        std::vector<Candidate> part(e - b);
+        #pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t i = b; i < e; ++i) {
+            const TreeNode child{tn.level - 1, L.members[i]};
+            Candidate best{TreeNode{0, 0}, 100000.0f};
+            if (!is_root[child.level][child.nix]) {
+                if (vol[child.level][child.nix] < cluster_vol) {
+                    float pen = edge_penalty(child, cluster_cut, cluster_vol);
+                    if (pen < best.penalty) best = {child, pen};
+                }
+                Candidate cd = candidate(child, label, cluster_cut, cluster_vol);
+                if (cd.penalty < best.penalty) best = cd;
+            }
+            part[i - b] = best;
+        }
+        Candidate best{TreeNode{0, 0}, 100000.0f};
+        for (const Candidate& c : part)
+            if (c.penalty < best.penalty) best = c;
         return best;
     }
 
@@ -192,20 +241,22 @@ public:
         for (const Level& L : h->levels) {
             vol.push_back(L.volumes);
             cut.push_back(L.cut_sizes);
+            is_root.emplace_back(L.nodes, 0);
         }
     }
 
     // normalized_cut.h:108-137
     const std::vector<int32_t>& compute(uint32_t k) {
-        roots.insert(TreeNode{static_cast<uint32_t>(h->levels.size() - 1), 0});
+        add_root(TreeNode{static_cast<uint32_t>(h->levels.size() - 1), 0});
         for (uint32_t i = 1; i <= k; ++i) {
+            write_labels = i == k;
             Candidate best{TreeNode{0, 0}, 1000.0f};
             int32_t root_ix = 0;
             float best_delta = 100000.0f;
             for (const TreeNode& root : roots) {
                 const uint32_t root_vol = vol[root.level][root.nix];
                 const uint32_t root_cut = cut[root.level][root.nix];
-                Candidate cd = candidate(root, root_ix, root_cut, root_vol);
+                Candidate cd = root_candidate(root, root_ix, root_cut, root_vol);
                 const float root_bias = as_frac(root_cut) / as_frac(root_vol);
                 const float delta = cd.penalty - root_bias;
                 if (delta < best_delta) {
@@ -215,7 +266,7 @@ public:
                 root_ix++;
             }
             split(best.tn);
-            roots.insert(best.tn);
+            add_root(best.tn);
         }
         return labels;
     }
@@ -269,6 +320,7 @@ int pxh_build(pxh_hierarchy* h, const pxd_params* params, float target_sparsity,
         const std::vector<uint32_t>& row = h->top_ranges();
         const std::vector<uint32_t>& nbr = h->top_edges();
         uint64_t cut_arcs = 0;
+        #pragma omp parallel for schedule(static, 4096) reduction(+ : cut_arcs)
         for (uint32_t v = 0; v < n; ++v)
             for (uint32_t a = row[v]; a < row[v + 1]; ++a) cut_arcs += ids[v] != ids[nbr[a]];
         const uint32_t cut_edges = static_cast<uint32_t>(cut_arcs / 2), m = row[n] / 2;
@@ -337,10 +389,14 @@ int pxh_normalized_cut(pxh_hierarchy* h, uint32_t k, int32_t* labels, float* val
     if (value) {
         // partition.h:418-427 over the parts in label order
         std::vector<uint32_t> pvol(k, 0), pint(k, 0);
+        uint32_t* pv = pvol.data();
+        uint32_t* pi = pint.data();
+        // u32 sums wrap like the reference's and are order-free
+        #pragma omp parallel for schedule(static, 4096) reduction(+ : pv[:k], pi[:k])
         for (uint32_t v = 0; v < h->n; ++v) {
             const uint32_t p = static_cast<uint32_t>(lab[v]);
-            pvol[p] += h->ranges[v + 1] - h->ranges[v];
-            for (uint32_t a = h->ranges[v]; a < h->ranges[v + 1]; ++a) pint[p] += lab[h->edges[a]] == lab[v];
+            pv[p] += h->ranges[v + 1] - h->ranges[v];
+            for (uint32_t a = h->ranges[v]; a < h->ranges[v + 1]; ++a) pi[p] += lab[h->edges[a]] == lab[v];
         }
         float nc = 0.0f;
         for (uint32_t p = 0; p < k; ++p)
