@@ -215,3 +215,21 @@ def test_gpu_hierarchy_levels_are_the_oracle_decompositions(name):
                 if vol[p]:
                     want = np.float32(want + np.float32(vol[p] - inner[p]) / np.float32(vol[p]))
             assert np.float32(value) == want
+
+
+@pytest.mark.gpu
+def test_gpu_hierarchy_of_the_torus_workload_matches_oracle():
+    """The hierarchy bench workload at full size (torus 1024^2 + 0.5 %, 2.1 M
+    edges, six levels): every level equals the oracle's decomposition."""
+    g = px.torus(1024, 10486, 1)
+    h = px.ExpanderHierarchy(g)
+    h.build()
+    parts = _oracle_levels(g)
+    assert h.height() == len(parts)
+    for li, (ids, k) in enumerate(parts):
+        L = h.lvl(li)
+        assert L.graph.n == k and np.array_equal(L.parents, ids), li
+    assert h.lvl(h.height() - 1).graph.n == 1
+    labels, value = px.NormalizedCut(h).compute(16)
+    size, vol, inner = _part_stats(g, labels, 16)
+    assert size.sum() == g.n and vol.sum() == 2 * g.m
