@@ -34,6 +34,7 @@ const char* pxh_last_error(void);
 /* ExpanderHierarchy(Graph*) (expander_hierarchy.h:105): keeps a copy of the
  * input CSR (n nodes, m undirected edges, 2m arcs). */
 int pxh_create(uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges, pxh_hierarchy** out);
+/* Frees the hierarchy and the engine its last pxh_build used. */
 int pxh_destroy(pxh_hierarchy* h);
 
 /* ExpanderHierarchy::build<true> (expander_hierarchy.h:107-140) with the engine
@@ -43,7 +44,9 @@ int pxh_destroy(pxh_hierarchy* h);
  * otherwise contract and push a level.  Stops when the top graph has at most
  * one node, or no edges (the reference loops forever there), and fails after
  * max_retries sparsity cuts in a row.  params may be NULL (engine defaults).
- * Levels pushed before a failure are kept. */
+ * One engine serves every level (each pxd_initialize reuses the device
+ * buffers of the level before) and is kept until pxh_destroy or the next
+ * pxh_build.  Levels pushed before a failure are kept. */
 int pxh_build(pxh_hierarchy* h, const pxd_params* params, float target_sparsity, float rw_threshold,
               uint32_t max_retries);
 

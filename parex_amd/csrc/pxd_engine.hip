@@ -104,6 +104,10 @@ struct pxd_engine {
     int dev = 0;
     hipStream_t st = nullptr;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;                 // size of each entry of allocs
+    // buffers of the previous input, kept for the next setup on this engine:
+    // (bytes, pointer), taken back best-fit; the rest is freed once setup ends
+    std::vector<std::pair<size_t, void*>> pool;
 
     // sizes
     uint32_t G = 0, N = 0, A = 0;
@@ -211,6 +215,7 @@ struct pxd_engine {
     pxd_engine() { std::fill(&tuned[0][0], &tuned[0][0] + 66, -1); }
     ~pxd_engine() {
         release();
+        trim_pool();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (tev0) (void)hipEventDestroy(tev0);
@@ -218,10 +223,16 @@ struct pxd_engine {
         if (st) (void)hipStreamDestroy(st);
     }
 
+    // The input's buffers go to the pool.  Nothing queued on the engine's
+    // stream may still use one when it is handed out again, so the stream is
+    // drained first (hipFree used to do that).  Every buffer is written by
+    // setup and init_state before it is read, as on a fresh allocation.
     void release() {
         stop_ranker();
-        for (void* p : allocs) (void)hipFree(p);
+        if (!allocs.empty() && st) (void)hipStreamSynchronize(st);
+        for (size_t i = 0; i < allocs.size(); ++i) pool.emplace_back(alloc_bytes[i], allocs[i]);
         allocs.clear();
+        alloc_bytes.clear();
         lvl_sum.clear();
         lvl_scan.clear();
         d_hcnt = d_hoff = d_hrow = nullptr;   // heavy-row buffers are allocated lazily
@@ -234,14 +245,34 @@ struct pxd_engine {
         inited = false;
     }
 
+    void trim_pool() {
+        for (auto& b : pool) (void)hipFree(b.second);
+        pool.clear();
+    }
+
     template <typename T>
     int alloc(T** p, size_t count) {
         void* q = nullptr;
-        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-        hipError_t e = hipMalloc(&q, bytes);
-        if (e != hipSuccess)
-            return set_err(PXD_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        size_t have = bytes, best = pool.size();
+        for (size_t i = 0; i < pool.size(); ++i)
+            if (pool[i].first >= bytes && (best == pool.size() || pool[i].first < pool[best].first)) best = i;
+        if (best < pool.size()) {
+            have = pool[best].first;
+            q = pool[best].second;
+            pool.erase(pool.begin() + best);
+        } else {
+            hipError_t e = hipMalloc(&q, bytes);
+            if (e != hipSuccess && !pool.empty()) {   // give the kept buffers back and retry
+                (void)hipGetLastError();
+                trim_pool();
+                e = hipMalloc(&q, bytes);
+            }
+            if (e != hipSuccess)
+                return set_err(PXD_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        }
         allocs.push_back(q);
+        alloc_bytes.push_back(have);
         *p = (T*)q;
         return 0;
     }
@@ -1555,19 +1586,25 @@ int pxd_initialize(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges
     if (!e || !ranges || (m && !edges)) return set_err(PXD_ERR_ARG, "null argument");
     const uint32_t* r[1] = {ranges};
     const uint32_t* ed[1] = {edges};
-    return e->setup(1, &n, &m, r, ed);
+    const int rc = e->setup(1, &n, &m, r, ed);
+    e->trim_pool();   // buffers of the previous input that this one did not reuse
+    return rc;
 }
 
 int pxd_initialize_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx) {
     if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    return e->setup_edges(n, m, fx, tx);
+    const int rc = e->setup_edges(n, m, fx, tx);
+    e->trim_pool();
+    return rc;
 }
 
 int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* node_counts, const uint32_t* edge_counts,
                          const uint32_t* const* ranges_ptrs, const uint32_t* const* edges_ptrs) {
     if (!e || !node_counts || !edge_counts || !ranges_ptrs || !edges_ptrs || num_graphs == 0)
         return set_err(PXD_ERR_ARG, "null argument or empty batch");
-    return e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
+    const int rc = e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
+    e->trim_pool();
+    return rc;
 }
 
 int pxd_set_thresholds(pxd_engine* e, float sc_threshold, float rw_threshold) {

@@ -13,7 +13,6 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
-#include <memory>
 #include <set>
 #include <string>
 #include <vector>
@@ -44,6 +43,14 @@ struct pxh_hierarchy {
     uint32_t n = 0;
     std::vector<uint32_t> ranges, edges;   // the input graph
     std::vector<Level> levels;
+    // the engine of the last pxh_build: every level re-initializes it, so the
+    // device buffers of the first (largest) level serve all later ones; freed
+    // by pxh_destroy
+    pxd_engine* engine = nullptr;
+
+    ~pxh_hierarchy() {
+        if (engine) pxd_destroy(engine);
+    }
 
     uint32_t top_nodes() const { return levels.empty() ? n : levels.back().nodes; }
     const std::vector<uint32_t>& top_ranges() const { return levels.empty() ? ranges : levels.back().ranges; }
@@ -111,15 +118,13 @@ int contract(pxh_hierarchy* h, uint32_t k, const uint32_t* member_off, const uin
     return 0;
 }
 
-// One GPU decomposition of the top graph: dense cluster ids.
-int decompose(const pxh_hierarchy* h, const pxd_params* params, float sc, float rw, std::vector<int32_t>& ids,
-              int32_t& k) {
+// One GPU decomposition of the top graph on the hierarchy's engine: dense
+// cluster ids.
+int decompose(const pxh_hierarchy* h, float sc, float rw, std::vector<int32_t>& ids, int32_t& k) {
     const uint32_t n = h->top_nodes();
     const std::vector<uint32_t>& row = h->top_ranges();
-    pxd_engine* e = nullptr;
-    int rc = pxd_create(params, &e);
-    if (rc) { g_err = std::string("pxd_create: ") + pxd_last_error(); return rc; }
-    std::unique_ptr<pxd_engine, int (*)(pxd_engine*)> guard(e, pxd_destroy);
+    pxd_engine* e = h->engine;
+    int rc = 0;
     int32_t it = 0;
     ids.assign(n, 0);
     if ((rc = pxd_set_thresholds(e, sc, rw)) ||
@@ -309,13 +314,22 @@ int pxh_build(pxh_hierarchy* h, const pxd_params* params, float target_sparsity,
     pxd_params p;
     if (params) p = *params;
     else pxd_default_params(&p);
+    if (h->engine) {
+        pxd_destroy(h->engine);
+        h->engine = nullptr;
+    }
+    if (int rc = pxd_create(&p, &h->engine)) {
+        h->engine = nullptr;
+        g_err = std::string("pxd_create: ") + pxd_last_error();
+        return rc;
+    }
     float sparsity = target_sparsity;
     uint32_t retries = 0;
     std::vector<int32_t> ids;
     // expander_hierarchy.h:113-132 (do ... while top graph has more than one node)
     while (h->top_nodes() > 1 && h->top_ranges().back() > 0) {
         int32_t k = 0;
-        if (int rc = decompose(h, &p, sparsity, rw_threshold, ids, k)) return rc;
+        if (int rc = decompose(h, sparsity, rw_threshold, ids, k)) return rc;
         const uint32_t n = h->top_nodes();
         const std::vector<uint32_t>& row = h->top_ranges();
         const std::vector<uint32_t>& nbr = h->top_edges();

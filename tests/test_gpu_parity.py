@@ -445,3 +445,43 @@ def test_tuned_engine_matches_oracle_on_a_large_sbm():
     ids, k = dm.getFinalPartition()
     oids, ok = ox.final_partition()
     assert k == ok and np.array_equal(ids, oids)
+
+
+def test_reinitialized_engine_equals_fresh_engines():
+    """One engine initialized again and again (its buffers are kept and handed
+    back to the next input): every result equals a fresh engine's, through a
+    larger input after smaller ones, a batch and an edge-list upload."""
+    seq = [px.sbm(30000, 12, 20 / 2499, 1 / 27500, 31), px.torus(64, 40, 3), px.rmat(14, 8, 0.57, 0.19, 0.19, 5),
+           px.sbm(60000, 20, 16 / 2999, 1 / 57000, 32)]
+    dm = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
+    for g in seq:
+        dm.initialize(g)
+        it = dm.expanderDecomposition()
+        ids, k = dm.getFinalPartition()
+        fresh = _engine(g, keep_original=False, record_node_data=False)
+        assert fresh.expanderDecomposition() == it
+        fids, fk = fresh.getFinalPartition()
+        assert fk == k and np.array_equal(fids, ids)
+        fresh.close()
+    dm.initialize_batch(seq[:3])
+    dm.expanderDecomposition()
+    bids, bk = dm.getFinalPartition()
+    off = 0
+    for i, g in enumerate(seq[:3]):
+        fresh = _engine(g, keep_original=False, record_node_data=False)
+        fresh.expanderDecomposition()
+        fids, fk = fresh.getFinalPartition()
+        assert fk == bk[i] and np.array_equal(fids, bids[off:off + g.n])
+        off += g.n
+        fresh.close()
+    g = seq[1]
+    src = np.repeat(np.arange(g.n), np.diff(g.ranges))
+    keep = src < g.edges
+    dm.initialize_edges(g.n, src[keep], g.edges[keep])
+    dm.expanderDecomposition()
+    eids, ek = dm.getFinalPartition()
+    fresh = _engine(g, keep_original=False, record_node_data=False)
+    fresh.expanderDecomposition()
+    fids, fk = fresh.getFinalPartition()
+    assert ek == fk and np.array_equal(eids, fids)
+    dm.close()
