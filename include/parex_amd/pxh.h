@@ -69,6 +69,11 @@ int pxh_level_size(pxh_hierarchy* h, uint32_t level, uint32_t* nodes, uint64_t* 
 int pxh_level(pxh_hierarchy* h, uint32_t level, uint32_t* ranges, uint32_t* edges, uint32_t* volumes,
               uint32_t* cut_sizes, uint32_t* parents, uint32_t* member_off, uint32_t* members);
 
+/* ExpanderHierarchy::subtractVolume / addCut (expander_hierarchy.h:154-159):
+ * u32 arithmetic on the volume or cut size of node nix of level l. */
+int pxh_subtract_volume(pxh_hierarchy* h, uint32_t level, uint32_t nix, uint32_t volume);
+int pxh_add_cut(pxh_hierarchy* h, uint32_t level, uint32_t nix, uint32_t cut);
+
 /* NormalizedCut::compute(k) (normalized_cut.h:108-144): k greedy splits of the
  * hierarchy tree from its single top node, in the reference's float
  * arithmetic and tie order.  labels[n] receives each input node's part in

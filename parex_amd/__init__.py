@@ -619,6 +619,8 @@ def pxh() -> C.CDLL:
                                        C.POINTER(C.c_uint32)]
         lib.pxh_level.argtypes = [P, C.c_uint32] + [_u32p] * 7
         lib.pxh_normalized_cut.argtypes = [P, C.c_uint32, _i32p, C.POINTER(C.c_float)]
+        lib.pxh_subtract_volume.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+        lib.pxh_add_cut.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
         lib._typed = True
     return lib
 
@@ -626,7 +628,7 @@ def pxh() -> C.CDLL:
 # C-ABI entry points declared in include/parex_amd/pxh.h (checked by the tests)
 PXH_SYMBOLS = [
     "pxh_last_error", "pxh_create", "pxh_destroy", "pxh_build", "pxh_add_level", "pxh_height",
-    "pxh_level_size", "pxh_level", "pxh_normalized_cut",
+    "pxh_level_size", "pxh_level", "pxh_normalized_cut", "pxh_subtract_volume", "pxh_add_cut",
 ]
 
 
@@ -700,7 +702,16 @@ class ExpanderHierarchy:
         ranges, edges, vol, cut, parents, off, members = a
         return Level(Graph(k, arcs // 2, ranges, edges[:arcs]), vol, cut, parents, off, members)
 
+    # expander_hierarchy.h:154-159
+    def subtractVolume(self, level: int, nix: int, vol: int):
+        self._check(self._lib.pxh_subtract_volume(self._h, level, nix, vol))
+
+    def addCut(self, level: int, nix: int, cut: int):
+        self._check(self._lib.pxh_add_cut(self._h, level, nix, cut))
+
     get_graph = getGraph
+    subtract_volume = subtractVolume
+    add_cut = addCut
 
 
 class NormalizedCut:

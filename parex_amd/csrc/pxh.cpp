@@ -392,6 +392,18 @@ int pxh_level(pxh_hierarchy* h, uint32_t level, uint32_t* ranges, uint32_t* edge
     return 0;
 }
 
+int pxh_subtract_volume(pxh_hierarchy* h, uint32_t level, uint32_t nix, uint32_t volume) {
+    if (!h || level >= h->levels.size() || nix >= h->levels[level].nodes) return fail("pxh_subtract_volume: no such node");
+    h->levels[level].volumes[nix] -= volume;
+    return 0;
+}
+
+int pxh_add_cut(pxh_hierarchy* h, uint32_t level, uint32_t nix, uint32_t cut) {
+    if (!h || level >= h->levels.size() || nix >= h->levels[level].nodes) return fail("pxh_add_cut: no such node");
+    h->levels[level].cut_sizes[nix] += cut;
+    return 0;
+}
+
 int pxh_normalized_cut(pxh_hierarchy* h, uint32_t k, int32_t* labels, float* value) {
     if (!h || !labels) return fail("pxh_normalized_cut: null argument");
     if (h->levels.empty() || h->levels.back().nodes != 1)

@@ -113,6 +113,29 @@ def test_contraction_of_a_small_graph_by_hand():
     assert np.float32(value) == np.float32(np.float32(1) / np.float32(7)) + np.float32(np.float32(1) / np.float32(9))
 
 
+def test_subtract_volume_and_add_cut_change_the_next_cut():
+    """expander_hierarchy.h:154-159: the level arrays change, and the next
+    normalized cut sees the change (it copies the arrays when it starts)."""
+    g = px.from_edges(7, [0, 1, 0, 2, 3, 4, 3, 5], [1, 2, 2, 3, 4, 5, 5, 6])
+    h = px.ExpanderHierarchy(g)
+    h.add_level(np.array([0, 3, 7]), np.array([0, 1, 2, 3, 4, 5, 6]))
+    h.add_level(np.array([0, 2]), np.array([0, 1]))
+    # both clusters cost 1/7 + 1/9 (float sums in either order): the first wins
+    labels, before = px.NormalizedCut(h).compute(2)
+    assert labels.tolist() == [0, 0, 0, 1, 1, 1, 1]
+    h.subtractVolume(0, 1, 2)
+    h.addCut(0, 0, 5)
+    L = h.lvl(0)
+    assert L.volumes.tolist() == [7, 7] and L.cut_sizes.tolist() == [6, 1]
+    with pytest.raises(px.PxdError):
+        h.addCut(0, 2, 1)
+    # now cluster 0 costs 6/7 + 6/9 and cluster 1 costs 1/7 + 1/9: cluster 1
+    # becomes the first root in (level, node) order and takes label 0
+    labels, after = px.NormalizedCut(h).compute(2)
+    assert labels.tolist() == [1, 1, 1, 0, 0, 0, 0]
+    assert after == before   # the objective is taken on the input graph
+
+
 def test_add_level_rejects_a_non_partition():
     g = px.from_edges(4, [0, 1, 2], [1, 2, 3])
     h = px.ExpanderHierarchy(g)
