@@ -228,7 +228,12 @@ def test_gpu_hierarchy_levels_are_the_oracle_decompositions(name):
     # level volumes add up to the input volume at every level
     for li in range(h.height()):
         assert int(h.lvl(li).volumes.astype(np.int64).sum()) == 2 * g.m
-    if top.n == 1:
+    if top.n != 1:
+        # a disconnected input ends with several top nodes and no edges; the
+        # reference's cut needs one top node
+        with pytest.raises(px.PxdError):
+            px.NormalizedCut(h).compute(2)
+    else:
         for k in (2, 8):
             labels, value = px.NormalizedCut(h).compute(k)
             size, vol, inner = _part_stats(g, labels, k)
