@@ -1074,8 +1074,10 @@ struct pxd_engine {
         const bool win = wv >= WV_WIN1 && (walk_force >= 0 || nl >= (1u << 20));
         for (uint32_t r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
             if (win) {
-                const uint32_t grid = (std::min<uint32_t>(cdiv(nl, 4096), 4096u) + NXCD - 1) / NXCD * NXCD;
-                k_cc_sample_win<20480, 4096><<<grid, WBS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part,
+                // 8192-entry chunks: the staged window (160 KB) is shared by twice
+                // the entries of a 4096 chunk (C4: 382 us instead of 419 us a launch)
+                const uint32_t grid = (std::min<uint32_t>(cdiv(nl, 8192), 4096u) + NXCD - 1) / NXCD * NXCD;
+                k_cc_sample_win<20480, 8192><<<grid, WBS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part,
                                                                    d_root, r, N);
             } else {
                 k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r);
