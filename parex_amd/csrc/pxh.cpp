@@ -13,7 +13,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <set>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -24,6 +27,21 @@ thread_local std::string g_err;
 int fail(const std::string& msg) {
     g_err = msg;
     return PXD_ERR_ARG;
+}
+
+// No C++ exception may cross the C ABI: host allocation failures become
+// PXD_ERR_OOM.
+template <class F>
+int guarded(const char* what, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        g_err = std::string(what) + ": out of host memory";
+        return PXD_ERR_OOM;
+    } catch (const std::exception& ex) {
+        g_err = std::string(what) + ": " + ex.what();
+        return PXD_ERR_STATE;
+    }
 }
 
 struct Level {
@@ -290,12 +308,14 @@ int pxh_create(uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* e
         if (ranges[v] > ranges[v + 1]) return fail("pxh_create: ranges is not ascending");
     for (uint64_t a = 0; a < 2ull * m; ++a)
         if (edges[a] >= n) return fail("pxh_create: an arc ends past n");
-    auto* h = new pxh_hierarchy;
-    h->n = n;
-    h->ranges.assign(ranges, ranges + n + 1);
-    h->edges.assign(edges, edges + 2ull * m);
-    *out = h;
-    return 0;
+    return guarded("pxh_create", [&] {
+        auto h = std::make_unique<pxh_hierarchy>();
+        h->n = n;
+        h->ranges.assign(ranges, ranges + n + 1);
+        h->edges.assign(edges, edges + 2ull * m);
+        *out = h.release();
+        return 0;
+    });
 }
 
 int pxh_destroy(pxh_hierarchy* h) {
@@ -305,12 +325,20 @@ int pxh_destroy(pxh_hierarchy* h) {
 
 int pxh_add_level(pxh_hierarchy* h, uint32_t num_clusters, const uint32_t* member_off, const uint32_t* members) {
     if (!h || !member_off || (h->top_nodes() && !members)) return fail("pxh_add_level: null argument");
-    return contract(h, num_clusters, member_off, members);
+    return guarded("pxh_add_level", [&] { return contract(h, num_clusters, member_off, members); });
 }
+
+static int build_levels(pxh_hierarchy* h, const pxd_params* params, float target_sparsity, float rw_threshold,
+                        uint32_t max_retries);
 
 int pxh_build(pxh_hierarchy* h, const pxd_params* params, float target_sparsity, float rw_threshold,
               uint32_t max_retries) {
     if (!h) return fail("pxh_build: null hierarchy");
+    return guarded("pxh_build", [&] { return build_levels(h, params, target_sparsity, rw_threshold, max_retries); });
+}
+
+static int build_levels(pxh_hierarchy* h, const pxd_params* params, float target_sparsity, float rw_threshold,
+                        uint32_t max_retries) {
     pxd_params p;
     if (params) p = *params;
     else pxd_default_params(&p);
@@ -404,11 +432,17 @@ int pxh_add_cut(pxh_hierarchy* h, uint32_t level, uint32_t nix, uint32_t cut) {
     return 0;
 }
 
+static int normalized_cut(pxh_hierarchy* h, uint32_t k, int32_t* labels, float* value);
+
 int pxh_normalized_cut(pxh_hierarchy* h, uint32_t k, int32_t* labels, float* value) {
     if (!h || !labels) return fail("pxh_normalized_cut: null argument");
     if (h->levels.empty() || h->levels.back().nodes != 1)
         return fail("pxh_normalized_cut: the hierarchy needs a top level with exactly one node");
     if (k == 0) return fail("pxh_normalized_cut: k must be at least 1");
+    return guarded("pxh_normalized_cut", [&] { return normalized_cut(h, k, labels, value); });
+}
+
+static int normalized_cut(pxh_hierarchy* h, uint32_t k, int32_t* labels, float* value) {
     NormalizedCutRun run(h);
     const std::vector<int32_t>& lab = run.compute(k);
     std::memcpy(labels, lab.data(), lab.size() * sizeof(int32_t));
