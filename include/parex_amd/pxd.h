@@ -212,7 +212,9 @@ int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n);
 int pxd_serve_wait(pxd_engine* e);
 /* Pop that result: cluster id per node (ids_out[n]), cluster count and the
  * iteration count its solo run reports (cuda/device.h:343-350), the ids
- * densely ranked as cuda/devPartition.h:798-823 does. */
+ * densely ranked as cuda/devPartition.h:798-823 does.  A failed result has a
+ * negative cluster count and no ids (n = 0 in pxd_serve_next): -1 when the
+ * graph failed the device check, -2 when the host had no memory to rank it. */
 int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations);
 int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
                     uint32_t* active_nodes, uint32_t* waiting_results);

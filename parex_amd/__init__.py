@@ -321,6 +321,8 @@ class Server:
         edges = np.ascontiguousarray(graph.edges, np.uint32)
         slot = C.c_int32()
         rc = self._check(self._lib.pxd_serve_admit(self._h, graph.n, graph.m, ranges, edges, tag, C.byref(slot)))
+        if rc == 0:
+            self._keep.append((ranges, edges))   # read by the device until the next step
         return rc == 0
 
     def set_walk_variant(self, variant: int):
@@ -683,7 +685,13 @@ class ExpanderHierarchy:
     # expander_hierarchy.h:29-88 for a given partition of the top graph
     def add_level(self, member_off: np.ndarray, members: np.ndarray):
         off = np.ascontiguousarray(member_off, np.uint32)
-        self._check(self._lib.pxh_add_level(self._h, len(off) - 1, off, np.ascontiguousarray(members, np.uint32)))
+        mem = np.ascontiguousarray(members, np.uint32)
+        h = self.height()
+        top = self.lvl(h - 1).graph.n if h else self._graph.n
+        if off.ndim != 1 or len(off) < 1 or mem.ndim != 1 or len(mem) != top:
+            raise ValueError(f"add_level: members must list each of the top level's {top} nodes once "
+                             f"and member_off hold num_clusters + 1 offsets")
+        self._check(self._lib.pxh_add_level(self._h, len(off) - 1, off, mem))
 
     def getGraph(self) -> Graph:
         return self._graph

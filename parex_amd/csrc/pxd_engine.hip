@@ -33,8 +33,27 @@ static int set_err(int code, const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    g_last_error = buf;
+    try {
+        g_last_error = buf;
+    } catch (...) {                   // no room for the message: keep the code
+    }
     return code;
+}
+
+// Every C entry point runs its body through this: host allocations (vectors
+// sized by the graph, the serving worker thread) may throw, and no exception
+// may cross the C ABI.  Allocation failures become PXD_ERR_OOM.
+template <class F>
+static int guarded(F&& f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return set_err(PXD_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& ex) {
+        return set_err(PXD_ERR_STATE, "%s", ex.what());
+    } catch (...) {
+        return set_err(PXD_ERR_STATE, "unexpected exception");
+    }
 }
 
 #define HIPC(x)                                                                                   \
@@ -517,7 +536,9 @@ struct pxd_engine {
 
     // back to the state right after initialize, from the device-resident input CSR
     int reset() {
-        if (!inited && h_gnode.empty()) return set_err(PXD_ERR_ARG, "engine not initialized");
+        // only a completed initialize leaves a valid device copy of the input:
+        // a failed one may have freed or never written it
+        if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (serving) return set_err(PXD_ERR_ARG, "a serving engine has no reset");
         if (!P.keep_original) return set_err(PXD_ERR_ARG, "reset needs keep_original = 1");
         HIPC(hipSetDevice(dev));
@@ -1136,9 +1157,10 @@ struct pxd_engine {
     std::vector<uint32_t> free_slots;
     uint32_t live_graphs = 0;
     uint64_t live_nodes = 0, live_arcs = 0;
+    static constexpr int32_t RANK_FAILED = -2;   // no host memory to rank the labels
     struct Done {
         uint64_t tag;
-        int32_t k, it;                // k < 0: the graph failed the device check
+        int32_t k, it;                // k = -1: the graph failed the device check; RANK_FAILED
         std::vector<int32_t> lab;     // raw labels, then cluster ids once ranked
         bool ranked;
         uint32_t n;                   // node count (lab is moved out while it is ranked)
@@ -1172,11 +1194,24 @@ struct pxd_engine {
             }
             std::vector<int32_t> lab = std::move(D->lab);   // deque elements keep their address
             lk.unlock();
-            std::vector<int32_t> ids(lab.size());
-            const int32_t k = dense_rank(lab.data(), (uint32_t)lab.size(), ids.data());
+            std::vector<int32_t> ids;
+            int32_t k = 0;
+            bool ok = true;
+            try {                     // an exception here would end the process
+                ids.resize(lab.size());
+                k = dense_rank(lab.data(), (uint32_t)lab.size(), ids.data());
+            } catch (...) {
+                ok = false;
+            }
             lk.lock();
-            D->lab = std::move(ids);
-            if (D->k >= 0) D->k = k;
+            if (ok) {
+                D->lab = std::move(ids);
+                if (D->k >= 0) D->k = k;
+            } else {                  // no room to rank: an error result without ids
+                D->lab.clear();
+                D->n = 0;
+                D->k = RANK_FAILED;
+            }
             D->ranked = true;
             done_cv.notify_all();
         }
@@ -1553,84 +1588,108 @@ void pxd_default_params(pxd_params* p) {
 const char* pxd_last_error(void) { return g_last_error.c_str(); }
 
 int pxd_create(const pxd_params* params, pxd_engine** out) {
-    if (!out) return set_err(PXD_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    pxd_engine* e = new (std::nothrow) pxd_engine();
-    if (!e) return set_err(PXD_ERR_OOM, "host allocation failed");
-    if (params) e->P = *params; else pxd_default_params(&e->P);
-    e->dev = e->P.device;
-    int ndev = 0;
-    hipError_t err = hipGetDeviceCount(&ndev);
-    if (err != hipSuccess || ndev <= e->dev) {
-        delete e;
-        return set_err(PXD_ERR_HIP, "no HIP device %d available (%s)", params ? params->device : 0,
-                       hipGetErrorString(err));
-    }
-    if (hipSetDevice(e->dev) != hipSuccess || hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-        hipEventCreate(&e->tev0) != hipSuccess || hipEventCreate(&e->tev1) != hipSuccess) {
-        delete e;
-        return set_err(PXD_ERR_HIP, "stream/event creation failed");
-    }
-    *out = e;
-    return 0;
+    return guarded([&]() -> int {
+        if (!out) return set_err(PXD_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        pxd_engine* e = new (std::nothrow) pxd_engine();
+        if (!e) return set_err(PXD_ERR_OOM, "host allocation failed");
+        if (params) e->P = *params; else pxd_default_params(&e->P);
+        e->dev = e->P.device;
+        int ndev = 0;
+        hipError_t err = hipGetDeviceCount(&ndev);
+        if (err != hipSuccess || ndev <= e->dev) {
+            delete e;
+            return set_err(PXD_ERR_HIP, "no HIP device %d available (%s)", params ? params->device : 0,
+                           hipGetErrorString(err));
+        }
+        if (hipSetDevice(e->dev) != hipSuccess || hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+            hipEventCreate(&e->tev0) != hipSuccess || hipEventCreate(&e->tev1) != hipSuccess) {
+            delete e;
+            return set_err(PXD_ERR_HIP, "stream/event creation failed");
+        }
+        *out = e;
+        return 0;
+    });
 }
 
 int pxd_destroy(pxd_engine* e) {
-    if (e) {
-        (void)hipSetDevice(e->dev);
-        delete e;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (e) {
+            (void)hipSetDevice(e->dev);
+            delete e;
+        }
+        return 0;
+    });
 }
 
 int pxd_initialize(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges) {
-    if (!e || !ranges || (m && !edges)) return set_err(PXD_ERR_ARG, "null argument");
-    const uint32_t* r[1] = {ranges};
-    const uint32_t* ed[1] = {edges};
-    const int rc = e->setup(1, &n, &m, r, ed);
-    e->trim_pool();   // buffers of the previous input that this one did not reuse
-    return rc;
+    return guarded([&]() -> int {
+        if (!e || !ranges || (m && !edges)) return set_err(PXD_ERR_ARG, "null argument");
+        const uint32_t* r[1] = {ranges};
+        const uint32_t* ed[1] = {edges};
+        const int rc = e->setup(1, &n, &m, r, ed);
+        e->trim_pool();   // buffers of the previous input that this one did not reuse
+        return rc;
+    });
 }
 
 int pxd_initialize_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    const int rc = e->setup_edges(n, m, fx, tx);
-    e->trim_pool();
-    return rc;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        const int rc = e->setup_edges(n, m, fx, tx);
+        e->trim_pool();
+        return rc;
+    });
 }
 
 int pxd_initialize_batch(pxd_engine* e, uint32_t num_graphs, const uint32_t* node_counts, const uint32_t* edge_counts,
                          const uint32_t* const* ranges_ptrs, const uint32_t* const* edges_ptrs) {
-    if (!e || !node_counts || !edge_counts || !ranges_ptrs || !edges_ptrs || num_graphs == 0)
-        return set_err(PXD_ERR_ARG, "null argument or empty batch");
-    const int rc = e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
-    e->trim_pool();
-    return rc;
+    return guarded([&]() -> int {
+        if (!e || !node_counts || !edge_counts || !ranges_ptrs || !edges_ptrs || num_graphs == 0)
+            return set_err(PXD_ERR_ARG, "null argument or empty batch");
+        const int rc = e->setup(num_graphs, node_counts, edge_counts, ranges_ptrs, edges_ptrs);
+        e->trim_pool();
+        return rc;
+    });
 }
 
 int pxd_set_thresholds(pxd_engine* e, float sc_threshold, float rw_threshold) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    e->P.sc_threshold = sc_threshold;
-    e->P.rw_threshold = rw_threshold;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        e->P.sc_threshold = sc_threshold;
+        e->P.rw_threshold = rw_threshold;
+        return 0;
+    });
 }
 
-int pxd_reset(pxd_engine* e) { return e ? e->reset() : set_err(PXD_ERR_ARG, "null engine"); }
+int pxd_reset(pxd_engine* e) {
+    return guarded([&]() -> int { return e ? e->reset() : set_err(PXD_ERR_ARG, "null engine"); });
+}
 
-int pxd_iterate_random_walk(pxd_engine* e) { return e ? e->walk() : set_err(PXD_ERR_ARG, "null engine"); }
-int pxd_compute_sweep_cuts(pxd_engine* e) { return e ? e->sweep() : set_err(PXD_ERR_ARG, "null engine"); }
-int pxd_cut_clusters(pxd_engine* e) { return e ? e->cut() : set_err(PXD_ERR_ARG, "null engine"); }
+int pxd_iterate_random_walk(pxd_engine* e) {
+    return guarded([&]() -> int { return e ? e->walk() : set_err(PXD_ERR_ARG, "null engine"); });
+}
+int pxd_compute_sweep_cuts(pxd_engine* e) {
+    return guarded([&]() -> int { return e ? e->sweep() : set_err(PXD_ERR_ARG, "null engine"); });
+}
+int pxd_cut_clusters(pxd_engine* e) {
+    return guarded([&]() -> int { return e ? e->cut() : set_err(PXD_ERR_ARG, "null engine"); });
+}
 
 int pxd_expander_decomposition(pxd_engine* e, int32_t* iterations) {
-    return e ? e->run(iterations) : set_err(PXD_ERR_ARG, "null engine");
+    return guarded([&]() -> int {
+        return e ? e->run(iterations) : set_err(PXD_ERR_ARG, "null engine");
+    });
 }
 
 int pxd_graph_iterations(pxd_engine* e, int32_t* out, uint32_t num_graphs) {
-    if (!e || !e->inited || !out || num_graphs < e->G) return set_err(PXD_ERR_ARG, "bad arguments");
-    HIPC(hipSetDevice(e->dev));
-    HIPC(hipMemcpy(out, e->d_giter, e->G * 4, hipMemcpyDeviceToHost));
-    return 0;
+    return guarded([&]() -> int {
+        if (!e || !e->inited || !out || num_graphs < e->G) return set_err(PXD_ERR_ARG, "bad arguments");
+        HIPC(hipSetDevice(e->dev));
+        HIPC(hipMemcpy(out, e->d_giter, e->G * 4, hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 static int require(pxd_engine* e) {
@@ -1641,57 +1700,69 @@ static int require(pxd_engine* e) {
 }
 
 int pxd_get_final_partition(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters) {
-    CHK(require(e));
-    if ((e->N && !ids_out) || !num_clusters) return set_err(PXD_ERR_ARG, "null output");
-    if (e->serving) return set_err(PXD_ERR_ARG, "a serving engine returns results with pxd_serve_take");
-    std::vector<int32_t> lab(e->N);
-    if (e->N) HIPC(hipMemcpy(lab.data(), e->d_label, e->N * 4, hipMemcpyDeviceToHost));
-    // cuda/devPartition.h:798-823 per graph: sort, unique, rank
-    for (uint32_t g = 0; g < e->G; ++g) {
-        const uint32_t b = e->h_gnode[g], en = e->h_gnode[g + 1];
-        num_clusters[g] = dense_rank(lab.data() + b, en - b, ids_out + b);
-    }
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if ((e->N && !ids_out) || !num_clusters) return set_err(PXD_ERR_ARG, "null output");
+        if (e->serving) return set_err(PXD_ERR_ARG, "a serving engine returns results with pxd_serve_take");
+        std::vector<int32_t> lab(e->N);
+        if (e->N) HIPC(hipMemcpy(lab.data(), e->d_label, e->N * 4, hipMemcpyDeviceToHost));
+        // cuda/devPartition.h:798-823 per graph: sort, unique, rank
+        for (uint32_t g = 0; g < e->G; ++g) {
+            const uint32_t b = e->h_gnode[g], en = e->h_gnode[g + 1];
+            num_clusters[g] = dense_rank(lab.data() + b, en - b, ids_out + b);
+        }
+        return 0;
+    });
 }
 
 int pxd_read_random_walk_values(pxd_engine* e, float* out) {
-    CHK(require(e));
-    std::vector<double> v(e->N);
-    if (e->N) HIPC(hipMemcpy(v.data(), e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < e->N; ++i) out[i] = (float)v[i];
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        std::vector<double> v(e->N);
+        if (e->N) HIPC(hipMemcpy(v.data(), e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < e->N; ++i) out[i] = (float)v[i];
+        return 0;
+    });
 }
 
 int pxd_read_random_walk_values_f64(pxd_engine* e, double* out) {
-    CHK(require(e));
-    if (e->N) HIPC(hipMemcpy(out, e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if (e->N) HIPC(hipMemcpy(out, e->d_x[e->cur], e->N * 8, hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 int pxd_set_random_walk_values_f64(pxd_engine* e, const double* values) {
-    CHK(require(e));
-    if (!e->N) return 0;
-    double* tmp = nullptr;
-    HIPC(hipMalloc(&tmp, (size_t)e->N * 8));
-    HIPC(hipMemcpy(tmp, values, (size_t)e->N * 8, hipMemcpyHostToDevice));
-    k_set_all<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, tmp, e->d_dint, e->d_x[e->cur], e->d_y[e->cur], e->d_xkey);
-    HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(e->st));
-    HIPC(hipFree(tmp));
-    e->sweep_done = false;            // the sweep order belongs to the old values
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if (!e->N) return 0;
+        double* tmp = nullptr;
+        HIPC(hipMalloc(&tmp, (size_t)e->N * 8));
+        HIPC(hipMemcpy(tmp, values, (size_t)e->N * 8, hipMemcpyHostToDevice));
+        k_set_all<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, tmp, e->d_dint, e->d_x[e->cur], e->d_y[e->cur], e->d_xkey);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(e->st));
+        HIPC(hipFree(tmp));
+        e->sweep_done = false;            // the sweep order belongs to the old values
+        return 0;
+    });
 }
 
 int pxd_download_degrees(pxd_engine* e, uint32_t* out) {
-    CHK(require(e));
-    if (e->N) HIPC(hipMemcpy(out, e->d_dint, e->N * 4, hipMemcpyDeviceToHost));
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if (e->N) HIPC(hipMemcpy(out, e->d_dint, e->N * 4, hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 int pxd_download_labels(pxd_engine* e, int32_t* out) {
-    CHK(require(e));
-    if (e->N) HIPC(hipMemcpy(out, e->d_label, e->N * 4, hipMemcpyDeviceToHost));
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if (e->N) HIPC(hipMemcpy(out, e->d_label, e->N * 4, hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 // rebuild the reference's in-place INVALID_EDGE view: the valid prefix of a
@@ -1720,209 +1791,251 @@ static int edge_view(pxd_engine* e, std::vector<uint32_t>& row, std::vector<uint
 }
 
 int pxd_download_graph(pxd_engine* e, uint32_t* ranges_out, uint32_t* edges_out) {
-    CHK(require(e));
-    std::vector<uint32_t> row, orig;
-    std::vector<uint8_t> valid;
-    CHK(edge_view(e, row, orig, valid));
-    for (uint32_t g = 0; g < e->G; ++g) {
-        const uint32_t nb = e->h_gnode[g], ab = e->h_garc[g];
-        for (uint32_t u = nb; u <= e->h_gnode[g + 1]; ++u) ranges_out[u] = row[u] - ab;
-        for (uint32_t j = ab; j < e->h_garc[g + 1]; ++j) edges_out[j] = valid[j] ? orig[j] - nb : PXD_INVALID_EDGE;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        std::vector<uint32_t> row, orig;
+        std::vector<uint8_t> valid;
+        CHK(edge_view(e, row, orig, valid));
+        for (uint32_t g = 0; g < e->G; ++g) {
+            const uint32_t nb = e->h_gnode[g], ab = e->h_garc[g];
+            for (uint32_t u = nb; u <= e->h_gnode[g + 1]; ++u) ranges_out[u] = row[u] - ab;
+            for (uint32_t j = ab; j < e->h_garc[g + 1]; ++j) edges_out[j] = valid[j] ? orig[j] - nb : PXD_INVALID_EDGE;
+        }
+        return 0;
+    });
 }
 
 int pxd_download_active_edge_map(pxd_engine* e, int32_t* out) {
-    CHK(require(e));
-    std::vector<uint32_t> row, orig;
-    std::vector<uint8_t> valid;
-    CHK(edge_view(e, row, orig, valid));
-    for (uint32_t j = 0; j < e->A; ++j) out[j] = valid[j];
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        std::vector<uint32_t> row, orig;
+        std::vector<uint8_t> valid;
+        CHK(edge_view(e, row, orig, valid));
+        for (uint32_t j = 0; j < e->A; ++j) out[j] = valid[j];
+        return 0;
+    });
 }
 
 int pxd_download_partition(pxd_engine* e, pxd_node_data* out, uint32_t capacity, uint32_t* count) {
-    CHK(require(e));
-    if (!e->P.record_node_data) return set_err(PXD_ERR_ARG, "engine created with record_node_data = 0");
-    const uint32_t c = std::min(capacity, e->sweep_done ? e->na : 0u);
-    if (c) HIPC(hipMemcpy(out, e->d_nd, c * sizeof(pxd_node_data), hipMemcpyDeviceToHost));
-    if (count) *count = c;
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        if (!e->P.record_node_data) return set_err(PXD_ERR_ARG, "engine created with record_node_data = 0");
+        const uint32_t c = std::min(capacity, e->sweep_done ? e->na : 0u);
+        if (c) HIPC(hipMemcpy(out, e->d_nd, c * sizeof(pxd_node_data), hipMemcpyDeviceToHost));
+        if (count) *count = c;
+        return 0;
+    });
 }
 
 int pxd_read_sweep_cuts(pxd_engine* e, int32_t* cluster_ids, pxd_sweep_cut* cuts, uint32_t capacity, uint32_t* count) {
-    CHK(require(e));
-    const uint32_t c = std::min(capacity, e->nc);
-    ClusterTable& T = e->ct[e->ctc];
-    std::vector<int32_t> lab(c);
-    std::vector<float> sp(c);
-    std::vector<uint32_t> off(c);
-    if (c) {
-        HIPC(hipMemcpy(lab.data(), T.clabel, c * 4, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(sp.data(), e->d_csp, c * 4, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(off.data(), e->d_coff, c * 4, hipMemcpyDeviceToHost));
-    }
-    for (uint32_t i = 0; i < c; ++i) {
-        if (cluster_ids) cluster_ids[i] = lab[i];
-        if (cuts) { cuts[i].clusterId = lab[i]; cuts[i].sparsity = sp[i]; cuts[i].offset = off[i]; }
-    }
-    if (count) *count = c;
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        const uint32_t c = std::min(capacity, e->nc);
+        ClusterTable& T = e->ct[e->ctc];
+        std::vector<int32_t> lab(c);
+        std::vector<float> sp(c);
+        std::vector<uint32_t> off(c);
+        if (c) {
+            HIPC(hipMemcpy(lab.data(), T.clabel, c * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(sp.data(), e->d_csp, c * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(off.data(), e->d_coff, c * 4, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < c; ++i) {
+            if (cluster_ids) cluster_ids[i] = lab[i];
+            if (cuts) { cuts[i].clusterId = lab[i]; cuts[i].sparsity = sp[i]; cuts[i].offset = off[i]; }
+        }
+        if (count) *count = c;
+        return 0;
+    });
 }
 
 int pxd_get_num_cut_edges(pxd_engine* e, uint32_t* out) {
-    CHK(require(e));
-    HIPC(hipMemsetAsync(e->d_u64, 0, 8, e->st));
-    if (e->N) k_sum_u32<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, e->d_dint, e->d_u64);
-    HIPC(hipGetLastError());
-    unsigned long long s = 0;
-    HIPC(hipMemcpyAsync(&s, e->d_u64, 8, hipMemcpyDeviceToHost, e->st));
-    HIPC(hipStreamSynchronize(e->st));
-    *out = (uint32_t)(e->M_total - s / 2);
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        HIPC(hipMemsetAsync(e->d_u64, 0, 8, e->st));
+        if (e->N) k_sum_u32<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, e->d_dint, e->d_u64);
+        HIPC(hipGetLastError());
+        unsigned long long s = 0;
+        HIPC(hipMemcpyAsync(&s, e->d_u64, 8, hipMemcpyDeviceToHost, e->st));
+        HIPC(hipStreamSynchronize(e->st));
+        *out = (uint32_t)(e->M_total - s / 2);
+        return 0;
+    });
 }
 
 int pxd_download_active_nodes(pxd_engine* e, pxd_labeled_node* out, uint32_t capacity, uint32_t* count) {
-    CHK(require(e));
-    const uint32_t c = std::min(capacity, e->na);
-    std::vector<uint32_t> nix(c), clu(c);
-    std::vector<int32_t> lab(e->nc);
-    if (c) {
-        HIPC(hipMemcpy(nix.data(), e->d_pnix[0], c * 4, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(clu.data(), e->d_pclu[0], c * 4, hipMemcpyDeviceToHost));
-    }
-    if (e->nc) HIPC(hipMemcpy(lab.data(), e->ct[e->ctc].clabel, e->nc * 4, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < c; ++i) {
-        out[i].nix = nix[i] - e->h_gnode[0];
-        out[i].clusterId = clu[i] < e->nc ? lab[clu[i]] : -1;
-    }
-    if (count) *count = c;
-    return 0;
+    return guarded([&]() -> int {
+        CHK(require(e));
+        const uint32_t c = std::min(capacity, e->na);
+        std::vector<uint32_t> nix(c), clu(c);
+        std::vector<int32_t> lab(e->nc);
+        if (c) {
+            HIPC(hipMemcpy(nix.data(), e->d_pnix[0], c * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(clu.data(), e->d_pclu[0], c * 4, hipMemcpyDeviceToHost));
+        }
+        if (e->nc) HIPC(hipMemcpy(lab.data(), e->ct[e->ctc].clabel, e->nc * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < c; ++i) {
+            out[i].nix = nix[i] - e->h_gnode[0];
+            out[i].clusterId = clu[i] < e->nc ? lab[clu[i]] : -1;
+        }
+        if (count) *count = c;
+        return 0;
+    });
 }
 
 int pxd_get_counts(pxd_engine* e, uint32_t* num_active_nodes, uint32_t* num_active_clusters,
                    uint32_t* num_clusters_with_cut) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    if (num_active_nodes) *num_active_nodes = e->na;
-    if (num_active_clusters) *num_active_clusters = e->nc;
-    if (num_clusters_with_cut) *num_clusters_with_cut = e->ncut;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        if (num_active_nodes) *num_active_nodes = e->na;
+        if (num_active_clusters) *num_active_clusters = e->nc;
+        if (num_clusters_with_cut) *num_clusters_with_cut = e->ncut;
+        return 0;
+    });
 }
 
 int pxd_last_run_ms(pxd_engine* e, double* ms) {
-    if (!e || !ms) return set_err(PXD_ERR_ARG, "null argument");
-    *ms = e->last_run_ms;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e || !ms) return set_err(PXD_ERR_ARG, "null argument");
+        *ms = e->last_run_ms;
+        return 0;
+    });
 }
 
 int pxd_enable_walk_timing(pxd_engine* e, int enable) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    e->walk_timing = enable != 0;
-    e->walk_launches = 0;
-    e->walk_ms = 0.0;
-    e->walk_bytes = 0.0;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        e->walk_timing = enable != 0;
+        e->walk_launches = 0;
+        e->walk_ms = 0.0;
+        e->walk_bytes = 0.0;
+        return 0;
+    });
 }
 
 int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    if (launches) *launches = e->walk_launches;
-    if (total_ms) *total_ms = e->walk_ms;
-    if (alg_bytes) *alg_bytes = e->walk_bytes;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        if (launches) *launches = e->walk_launches;
+        if (total_ms) *total_ms = e->walk_ms;
+        if (alg_bytes) *alg_bytes = e->walk_bytes;
+        return 0;
+    });
 }
 
 int pxd_set_walk_variant(pxd_engine* e, int variant) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    if (variant < -2 || variant >= pxd_engine::WV_COUNT) return set_err(PXD_ERR_ARG, "walk variant %d out of range", variant);
-    e->walk_force = variant;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        if (variant < -2 || variant >= pxd_engine::WV_COUNT) return set_err(PXD_ERR_ARG, "walk variant %d out of range", variant);
+        e->walk_force = variant;
+        return 0;
+    });
 }
 
 int pxd_walk_tuning(pxd_engine* e, int32_t* table, uint32_t capacity, uint64_t* tunings) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    for (uint32_t b = 0; b < capacity && b < 66; ++b) table[b] = e->tuned[b / 33][b % 33];
-    if (tunings) *tunings = e->walk_tunings;
-    return 0;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        for (uint32_t b = 0; b < capacity && b < 66; ++b) table[b] = e->tuned[b / 33][b % 33];
+        if (tunings) *tunings = e->walk_tunings;
+        return 0;
+    });
 }
 
 int pxd_serve_create(const pxd_params* params, uint32_t node_capacity, uint64_t arc_capacity, uint32_t graph_slots,
                      pxd_engine** out) {
-    CHK(pxd_create(params, out));
-    const int rc = (*out)->serve_create(node_capacity, arc_capacity, graph_slots);
-    if (rc != 0) {
-        delete *out;
-        *out = nullptr;
-    }
-    return rc;
+    return guarded([&]() -> int {
+        CHK(pxd_create(params, out));
+        const int rc = (*out)->serve_create(node_capacity, arc_capacity, graph_slots);
+        if (rc != 0) {
+            delete *out;
+            *out = nullptr;
+        }
+        return rc;
+    });
 }
 
 int pxd_serve_admit(pxd_engine* e, uint32_t n, uint32_t m, const uint32_t* ranges, const uint32_t* edges,
                     uint64_t tag, int32_t* slot) {
-    return e ? e->serve_admit(n, m, ranges, edges, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+    return guarded([&]() -> int {
+        return e ? e->serve_admit(n, m, ranges, edges, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+    });
 }
 
 int pxd_serve_enable_edges(pxd_engine* e, uint32_t max_nodes, uint64_t max_edges) {
-    return e ? e->serve_enable_edges(max_nodes, max_edges) : set_err(PXD_ERR_ARG, "null engine");
+    return guarded([&]() -> int {
+        return e ? e->serve_enable_edges(max_nodes, max_edges) : set_err(PXD_ERR_ARG, "null engine");
+    });
 }
 
 int pxd_serve_admit_edges(pxd_engine* e, uint32_t n, uint64_t m, const uint32_t* fx, const uint32_t* tx, uint64_t tag,
                           int32_t* slot) {
-    return e ? e->serve_admit_edges(n, m, fx, tx, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+    return guarded([&]() -> int {
+        return e ? e->serve_admit_edges(n, m, fx, tx, tag, slot) : set_err(PXD_ERR_ARG, "null engine");
+    });
 }
 
 int pxd_serve_step(pxd_engine* e, uint32_t* finished) {
-    return e ? e->serve_step(finished) : set_err(PXD_ERR_ARG, "null engine");
+    return guarded([&]() -> int {
+        return e ? e->serve_step(finished) : set_err(PXD_ERR_ARG, "null engine");
+    });
 }
 
 int pxd_serve_next(pxd_engine* e, uint64_t* tag, uint32_t* n) {
-    if (!e || !tag || !n) return set_err(PXD_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(e->done_mu);
-    if (e->done.empty() || !e->done.front().ranked) return PXD_EMPTY;
-    *tag = e->done.front().tag;
-    *n = e->done.front().n;
-    return PXD_OK;
+    return guarded([&]() -> int {
+        if (!e || !tag || !n) return set_err(PXD_ERR_ARG, "null argument");
+        std::lock_guard<std::mutex> lk(e->done_mu);
+        if (e->done.empty() || !e->done.front().ranked) return PXD_EMPTY;
+        *tag = e->done.front().tag;
+        *n = e->done.front().n;
+        return PXD_OK;
+    });
 }
 
 int pxd_serve_wait(pxd_engine* e) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    std::unique_lock<std::mutex> lk(e->done_mu);
-    e->done_cv.wait(lk, [e] {
-        for (const pxd_engine::Done& d : e->done)
-            if (!d.ranked) return false;
-        return true;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        std::unique_lock<std::mutex> lk(e->done_mu);
+        e->done_cv.wait(lk, [e] {
+            for (const pxd_engine::Done& d : e->done)
+                if (!d.ranked) return false;
+            return true;
+        });
+        return PXD_OK;
     });
-    return PXD_OK;
 }
 
 int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32_t* iterations) {
-    if (!e || !num_clusters || !iterations) return set_err(PXD_ERR_ARG, "null argument");
-    std::unique_lock<std::mutex> lk(e->done_mu);
-    if (e->done.empty()) return PXD_EMPTY;
-    e->done_cv.wait(lk, [e] { return e->done.front().ranked; });
-    pxd_engine::Done& D = e->done.front();
-    if (D.n) {
-        if (!ids_out) return set_err(PXD_ERR_ARG, "null output");
-        std::memcpy(ids_out, D.lab.data(), (size_t)D.n * 4);
-    }
-    *num_clusters = D.k;
-    *iterations = D.it;
-    e->done.pop_front();
-    return PXD_OK;
+    return guarded([&]() -> int {
+        if (!e || !num_clusters || !iterations) return set_err(PXD_ERR_ARG, "null argument");
+        std::unique_lock<std::mutex> lk(e->done_mu);
+        if (e->done.empty()) return PXD_EMPTY;
+        e->done_cv.wait(lk, [e] { return e->done.front().ranked; });
+        pxd_engine::Done& D = e->done.front();
+        if (D.n) {
+            if (!ids_out) return set_err(PXD_ERR_ARG, "null output");
+            std::memcpy(ids_out, D.lab.data(), (size_t)D.n * 4);
+        }
+        *num_clusters = D.k;
+        *iterations = D.it;
+        e->done.pop_front();
+        return PXD_OK;
+    });
 }
 
 int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
                     uint32_t* active_nodes, uint32_t* waiting_results) {
-    if (!e) return set_err(PXD_ERR_ARG, "null engine");
-    if (live_graphs) *live_graphs = e->live_graphs;
-    if (live_nodes) *live_nodes = e->live_nodes;
-    if (live_arcs) *live_arcs = e->live_arcs;
-    if (active_nodes) *active_nodes = e->na;
-    if (waiting_results) {
-        std::lock_guard<std::mutex> lk(e->done_mu);
-        *waiting_results = (uint32_t)e->done.size();
-    }
-    return PXD_OK;
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        if (live_graphs) *live_graphs = e->live_graphs;
+        if (live_nodes) *live_nodes = e->live_nodes;
+        if (live_arcs) *live_arcs = e->live_arcs;
+        if (active_nodes) *active_nodes = e->na;
+        if (waiting_results) {
+            std::lock_guard<std::mutex> lk(e->done_mu);
+            *waiting_results = (uint32_t)e->done.size();
+        }
+        return PXD_OK;
+    });
 }
 
 }  // extern "C"

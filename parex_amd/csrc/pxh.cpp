@@ -85,11 +85,14 @@ int contract(pxh_hierarchy* h, uint32_t k, const uint32_t* member_off, const uin
     const std::vector<uint32_t>& row = h->top_ranges();
     const std::vector<uint32_t>& nbr = h->top_edges();
     if (member_off[0] != 0 || member_off[k] != nb) return fail("pxh_add_level: member_off must run from 0 to the node count");
+    // every offset checked before any member is read: ascending from 0 to nb
+    // keeps every read of members below nb
+    for (uint32_t c = 0; c < k; ++c)
+        if (member_off[c] > member_off[c + 1]) return fail("pxh_add_level: member_off is not ascending");
     Level L;
     L.nodes = k;
     L.parents.assign(nb, UINT32_MAX);
     for (uint32_t c = 0; c < k; ++c) {
-        if (member_off[c] > member_off[c + 1]) return fail("pxh_add_level: member_off is not ascending");
         for (uint32_t i = member_off[c]; i < member_off[c + 1]; ++i) {
             uint32_t v = members[i];
             if (v >= nb || L.parents[v] != UINT32_MAX) return fail("pxh_add_level: members is not a partition of the nodes");

@@ -141,8 +141,16 @@ def test_add_level_rejects_a_non_partition():
     h = px.ExpanderHierarchy(g)
     with pytest.raises(px.PxdError):
         h.add_level(np.array([0, 2, 4]), np.array([0, 1, 1, 3]))
-    with pytest.raises(px.PxdError):
+    # the binding checks that members lists every node once before the call
+    with pytest.raises(ValueError):
         h.add_level(np.array([0, 2, 3]), np.array([0, 1, 2]))
+    # offsets that overshoot the node count and come back: refused before any
+    # member is read (the library checks every offset first)
+    with pytest.raises(px.PxdError, match="not ascending"):
+        h.add_level(np.array([0, 7, 4]), np.array([0, 1, 2, 3]))
+    lib, hh = h._lib, h._h
+    off = np.array([0, 9, 4], np.uint32)
+    assert lib.pxh_add_level(hh, 2, off, np.array([0, 1, 2, 3], np.uint32)) != 0
     assert h.height() == 0
 
 
