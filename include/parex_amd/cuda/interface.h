@@ -1,8 +1,10 @@
 // Drop-in C++ host API of the expander decomposition: the same class, members
 // and types as the reference's cuda/interface.h:11-60, implemented over the
 // MI355X engine's C ABI (include/parex_amd/pxd.h) by parex_amd/csrc/facade.cpp.
-#ifndef PAREX_AMD_CUDA_INTERFACE_H
-#define PAREX_AMD_CUDA_INTERFACE_H
+// The include guard is the reference header's own (cuda/interface.h:5), so a
+// caller that reaches both declares the class once.
+#ifndef PAREX_INTERFACE_H
+#define PAREX_INTERFACE_H
 
 #include <memory>
 #include <vector>

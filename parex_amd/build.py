@@ -6,7 +6,8 @@ cross-compiles gfx950 code without a GPU).
   parex_amd/libpxh.so     expander hierarchy + normalized cut (g++, OpenMP, links libpxd)
   parex_amd/bin/parex     CLI drop-in (g++, links both)
 
-The oracle (test infrastructure) is built by oracle/Makefile.
+The oracle and the C++ test programs (test infrastructure) are built by
+oracle/Makefile and tests/cpp/Makefile.
 """
 from __future__ import annotations
 
@@ -72,6 +73,11 @@ def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "all"])
 
 
+def build_test_programs():
+    """tests/cpp: the CudaDeviceManager facade stepped against the oracle."""
+    _run(["make", "-C", os.path.join(ROOT, "tests", "cpp"), "-B", "all"])
+
+
 def build(oracle: bool = True):
     build_pxgen()
     build_engine()
@@ -79,6 +85,7 @@ def build(oracle: bool = True):
     build_hierarchy()
     if oracle:
         build_oracle()
+        build_test_programs()
 
 
 if __name__ == "__main__":

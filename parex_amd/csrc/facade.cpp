@@ -3,6 +3,7 @@
 // Parameters come from the process-wide globals exactly when the reference
 // reads them: randSeed at initialize(), the two thresholds on every step.
 #include <cstdio>
+#include <cstdlib>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -14,13 +15,26 @@ namespace {
 void check(int rc) {
     if (rc != 0) throw std::runtime_error(std::string("pxd: ") + pxd_last_error());
 }
-constexpr uint32_t kMaxIter = 200000;   // cuda/device.h:309
+constexpr uint32_t kMaxIter = 200000;   // MAX_NUM_ITER, cuda/device.h:309
+
+// The reference's cap is a constant; PAREX_MAX_ITER (a positive integer)
+// lowers or raises it for this process, e.g. to exercise the capped exit.
+uint32_t max_iterations() {
+    const char* s = std::getenv("PAREX_MAX_ITER");
+    if (!s || !*s) return kMaxIter;
+    char* end = nullptr;
+    const unsigned long v = std::strtoul(s, &end, 10);
+    if (*end != '\0' || v == 0 || v > 0x7FFFFFFFul)
+        throw std::runtime_error("PAREX_MAX_ITER must be a positive integer below 2^31");
+    return (uint32_t)v;
+}
 }  // namespace
 
 struct CudaDeviceManager::Impl {
     pxd_engine* e = nullptr;
     NodeIx n = 0;
     EdgeIx m = 0;
+    uint32_t max_iter = kMaxIter;
 
     ~Impl() { pxd_destroy(e); }
 
@@ -32,7 +46,8 @@ struct CudaDeviceManager::Impl {
         p.seed = randSeed;
         p.sc_threshold = sc_threshold;
         p.rw_threshold = rw_threshold;
-        p.max_iter = kMaxIter;
+        max_iter = max_iterations();
+        p.max_iter = max_iter;
         check(pxd_create(&p, &e));
     }
     pxd_engine* eng() {
@@ -113,7 +128,7 @@ void CudaDeviceManager::expanderDecomposition() {
     std::cout << "Begin Expander Decomposition" << std::endl;
     int32_t it = 0;
     check(pxd_expander_decomposition(impl->eng(), &it));
-    if ((uint32_t)it >= kMaxIter) printf("Not terminating. ABORT.\n");
+    if ((uint32_t)it >= impl->max_iter) printf("Not terminating. ABORT.\n");
     printf("==================================================================================== \n"
            "iterations: %d\n", it);
 }
