@@ -75,6 +75,20 @@ def serve_pool(seed=13):
     return {"sbm": sbm, "torus": tor, "rmat": rm}
 
 
+def serve_trace(pool, rate=200.0, dur=60.0, seed=13):
+    """Arrivals of the serving run: Poisson at `rate` per second for `dur`
+    seconds; 60/30/10 % SBM/torus/RMAT, each request a uniform pick among
+    the pool's graphs of its kind.  Returns the arrival times, the (kind,
+    index) picks and the generator (its later draws pick the probes)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 100)
+    arr = np.cumsum(gaps)
+    arr = arr[arr < dur]
+    kinds = rng.choice(["sbm", "torus", "rmat"], size=len(arr), p=[0.6, 0.3, 0.1])
+    picks = [(str(k), int(rng.integers(len(pool[k])))) for k in kinds]
+    return arr, picks, rng
+
+
 def pinned_pool(pool):
     """Copies of the request graphs in pinned host memory, as a server's
     receive buffers would be: uploads are then plain DMA."""
@@ -116,17 +130,13 @@ def edge_lists(pool):
 
 
 def serve_workload(a):
-    rng = np.random.default_rng(13)
     t = time.perf_counter()
     pool, _pinned = pinned_pool(serve_pool())
     epool, _epinned = edge_lists(pool) if a.admit == "edges" else (None, None)
     log(f"serving pool: {sum(len(v) for v in pool.values())} graphs ({time.perf_counter() - t:.1f} s)")
-    rate, dur = 200.0, a.duration
-    gaps = rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 100)
-    arr = np.cumsum(gaps)
-    arr = arr[arr < dur]
-    kinds = rng.choice(["sbm", "torus", "rmat"], size=len(arr), p=[0.6, 0.3, 0.1])
-    req = [pool[k][int(rng.integers(len(pool[k])))] for k in kinds]
+    dur = a.duration
+    arr, picks, rng = serve_trace(pool, 200.0, dur, 13)
+    req = [pool[k][i] for k, i in picks]
     offered = sum(g.m for g in req) / dur
     log(f"{len(req)} requests over {dur:.0f} s, offered {offered:.3e} edges/s")
     srv = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=4096,

@@ -5,6 +5,8 @@
  */
 #include "oracle.h"
 
+#include <omp.h>
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -44,7 +46,13 @@ struct ox_state {
     uint32_t* idx;
     uint32_t* tmp;
     uint32_t* uf;
+    double* y;            /* x[v] / dInt[v] of the current walk step */
+    uint64_t* nkey;       /* packed key of every node for the cut contribution */
 };
+
+/* OpenMP threads of the calling thread's later oracle calls (each per-node
+ * loop writes only its own node's outputs: any count gives the same result) */
+void ox_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 /* ------------------------------------------------------------------ init RNG */
 
@@ -198,34 +206,35 @@ float ox_cluster_sum(const float* vals, uint32_t k) {
 
 /* ------------------------------------------------------------- stable sorts */
 
-static void msort_u64(uint32_t* idx, uint32_t* tmp, uint32_t lo, uint32_t hi, const uint64_t* key) {
-    if (hi - lo < 2) return;
-    uint32_t mid = lo + (hi - lo) / 2;
-    msort_u64(idx, tmp, lo, mid, key);
-    msort_u64(idx, tmp, mid, hi, key);
-    uint32_t i = lo, j = mid, o = lo;
-    while (i < mid && j < hi) {
-        if (key[idx[j]] < key[idx[i]]) tmp[o++] = idx[j++];
-        else tmp[o++] = idx[i++];
+/* Stable sort of idx[0..n) by key[idx[i]] (LSD radix over 16-bit digits,
+ * digits constant over the input skipped).  Any stable sort gives the same
+ * order, which is what the reference's CUB radix sort produces. */
+static void rsort_u64(uint32_t* idx, uint32_t* tmp, uint32_t n, const uint64_t* key) {
+    if (n < 2) return;
+    uint64_t orv = 0, andv = ~0ull;
+    for (uint32_t i = 0; i < n; ++i) { orv |= key[idx[i]]; andv &= key[idx[i]]; }
+    const uint64_t vary = orv ^ andv;
+    uint32_t* cnt = (uint32_t*)malloc(65536u * 4u);
+    uint32_t *src = idx, *dst = tmp;
+    for (int sh = 0; sh < 64; sh += 16) {
+        if (((vary >> sh) & 0xFFFFu) == 0) continue;
+        memset(cnt, 0, 65536u * 4u);
+        for (uint32_t i = 0; i < n; ++i) ++cnt[(key[src[i]] >> sh) & 0xFFFFu];
+        uint32_t run = 0;
+        for (uint32_t d = 0; d < 65536u; ++d) { uint32_t c = cnt[d]; cnt[d] = run; run += c; }
+        for (uint32_t i = 0; i < n; ++i) dst[cnt[(key[src[i]] >> sh) & 0xFFFFu]++] = src[i];
+        uint32_t* t = src; src = dst; dst = t;
     }
-    while (i < mid) tmp[o++] = idx[i++];
-    while (j < hi) tmp[o++] = idx[j++];
-    memcpy(idx + lo, tmp + lo, (hi - lo) * sizeof(uint32_t));
+    if (src != idx) memcpy(idx, src, (size_t)n * sizeof(uint32_t));
+    free(cnt);
 }
 
-static void msort_lab(uint32_t* idx, uint32_t* tmp, uint32_t lo, uint32_t hi, const ox_lnode* a) {
-    if (hi - lo < 2) return;
-    uint32_t mid = lo + (hi - lo) / 2;
-    msort_lab(idx, tmp, lo, mid, a);
-    msort_lab(idx, tmp, mid, hi, a);
-    uint32_t i = lo, j = mid, o = lo;
-    while (i < mid && j < hi) {
-        if (a[idx[j]].clusterId < a[idx[i]].clusterId) tmp[o++] = idx[j++];
-        else tmp[o++] = idx[i++];
-    }
-    while (i < mid) tmp[o++] = idx[i++];
-    while (j < hi) tmp[o++] = idx[j++];
-    memcpy(idx + lo, tmp + lo, (hi - lo) * sizeof(uint32_t));
+/* stable sort of idx by a[idx[i]].clusterId (signed) */
+static void rsort_lab(uint32_t* idx, uint32_t* tmp, uint32_t n, const ox_lnode* a) {
+    uint64_t* k = (uint64_t*)malloc((size_t)(n ? n : 1) * 8);
+    for (uint32_t i = 0; i < n; ++i) k[i] = (uint64_t)((uint32_t)a[i].clusterId ^ 0x80000000u);
+    rsort_u64(idx, tmp, n, k);
+    free(k);
 }
 
 /* ------------------------------------------------------------- construction */
@@ -256,6 +265,8 @@ ox_state* ox_create(uint32_t n, uint32_t m, const uint32_t* ranges, const uint32
     s->idx = (uint32_t*)malloc(nn * 4);
     s->tmp = (uint32_t*)malloc(nn * 4);
     s->uf = (uint32_t*)malloc(nn * 4);
+    s->y = (double*)malloc(nn * 8);
+    s->nkey = (uint64_t*)malloc(nn * 8);
     memcpy(s->ranges, ranges, (size_t)(n + 1) * 4);
     if (m) memcpy(s->nbr, edges, 2ull * m * 4);
     /* GraphManager ctor, cuda/devGraph.h:125-139 */
@@ -287,6 +298,7 @@ void ox_destroy(ox_state* s) {
     free(s->dist[0]); free(s->dist[1]); free(s->act); free(s->keys); free(s->nd_pre);
     free(s->nd); free(s->clabels); free(s->cvol); free(s->cdata); free(s->lookup);
     free(s->scbuf); free(s->sccut); free(s->idx); free(s->tmp); free(s->uf);
+    free(s->y); free(s->nkey);
     free(s);
 }
 
@@ -296,27 +308,36 @@ void ox_destroy(ox_state* s) {
 void ox_walk(ox_state* s) {
     const double* x = s->dist[s->cur];
     double* xn = s->dist[s->cur ^ 1];
-    for (uint32_t i = 0; i < s->na; ++i) {
+    /* the term x[nb] / (double)dInt[nb] of every arc into nb, divided once per
+     * node (the same IEEE quotient; terms with dInt = 0 are skipped below) */
+    const int64_t n = s->n, na = s->na;
+    int err = 0;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; ++v) s->y[v] = s->dint[v] ? x[v] / (double)s->dint[v] : 0.0;
+    /* every position writes only its own node's value and its own key, and
+     * the neighbour sum is exact: any thread split gives the same bits */
+#pragma omp parallel for schedule(dynamic, 2048) reduction(| : err)
+    for (int64_t i = 0; i < na; ++i) {
         uint32_t nix = s->act[i].nix;
         int32_t lab = s->act[i].clusterId;
         if (lab < 0) { s->keys[i] = ox_pack_key(lab, 0.0f); continue; }
         uint32_t start = s->ranges[nix], end = start + s->deg[nix];
-        if (s->deg[nix] > (1u << 24)) s->err |= 2;
+        if (s->deg[nix] > (1u << 24)) err |= 2;
         i128 acc = 0;
         int ovf = 0;
         for (uint32_t j = start; j < end; ++j) {
             uint32_t nb = s->nbr[j];
             if (nb == OX_INVALID_EDGE) continue;
-            uint32_t d = s->dint[nb];
-            if (d == 0) { s->err |= 4; continue; }      /* reference prints ERROR and skips */
-            fix_add(&acc, x[nb] / (double)d, &ovf);
+            if (s->dint[nb] == 0) { err |= 4; continue; }   /* reference prints ERROR and skips */
+            fix_add(&acc, s->y[nb], &ovf);
         }
-        if (ovf) s->err |= 1;
+        if (ovf) err |= 1;
         double sum = fix_to_double(acc);
         double v = sum * (1.0 - RW_STAY) + x[nix] * RW_STAY;
         xn[nix] = v;
         s->keys[i] = ox_pack_key(lab, (float)v);
     }
+    s->err |= err;
     s->cur ^= 1;
 }
 
@@ -327,8 +348,15 @@ void ox_walk(ox_state* s) {
 void ox_sweep(ox_state* s) {
     const double* x = s->dist[s->cur];
     uint32_t na = s->na;
-    /* K4 nodeDiffKernel_Sparse_WarpParallel, cuda/devSweepCut.h:20-112 */
-    for (uint32_t i = 0; i < na; ++i) {
+    /* every neighbour's key (cuda/devSweepCut.h:78-81), packed once per node */
+    const int64_t nn = s->n, nna = na;
+    int err = 0;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < nn; ++v) s->nkey[v] = ox_pack_key(s->labels[v], (float)x[v]);
+    /* K4 nodeDiffKernel_Sparse_WarpParallel, cuda/devSweepCut.h:20-112 (each
+     * position writes only its own NodeData) */
+#pragma omp parallel for schedule(dynamic, 2048) reduction(| : err)
+    for (int64_t i = 0; i < nna; ++i) {
         uint32_t nix = s->act[i].nix;
         int32_t lab = s->act[i].clusterId;
         if (lab < 0) continue;                            /* stale NodeData stays */
@@ -338,8 +366,8 @@ void ox_sweep(ox_state* s) {
         for (uint32_t j = start; j < end; ++j) {
             uint32_t nb = s->nbr[j];
             if (nb == OX_INVALID_EDGE) continue;
-            uint64_t nk = ox_pack_key(s->labels[nb], (float)x[nb]);
-            if (s->labels[nb] != lab) s->err |= 8;        /* reference prints ERROR */
+            uint64_t nk = s->nkey[nb];
+            if (s->labels[nb] != lab) err |= 8;           /* reference prints ERROR */
             int before = (nk != myKey) ? (nk < myKey) : (nb < nix);
             c += before ? -1 : 1;
         }
@@ -348,9 +376,10 @@ void ox_sweep(ox_state* s) {
         s->nd_pre[i].prefixVolume = s->dint[nix];
         s->nd_pre[i].offsetInCluster = 1;
     }
+    s->err |= err;
     /* K5: stable LSD radix sort of (key, NodeData) pairs over all 64 bits */
     for (uint32_t i = 0; i < na; ++i) s->idx[i] = i;
-    msort_u64(s->idx, s->tmp, 0, na, s->keys);
+    rsort_u64(s->idx, s->tmp, na, s->keys);
     /* apply permutation to keys and NodeData */
     uint64_t* sk = (uint64_t*)malloc((size_t)(na ? na : 1) * 8);
     ox_ndata* snd = (ox_ndata*)malloc((size_t)(na ? na : 1) * sizeof(ox_ndata));
@@ -440,11 +469,14 @@ static uint32_t uf_find(uint32_t* p, uint32_t a) {
 void ox_cut(ox_state* s) {
     uint32_t na = s->na;
     /* K12 PartitionManager::cutClusters, cuda/devPartition.h:545-622 */
+    int err = 0;
     if (s->ncut > 0) {
         int32_t maxLabel = s->clabels[s->nac - 1];
-        for (uint32_t k = 0; k < na; ++k) {
+        /* each position updates only itself and its own node */
+#pragma omp parallel for schedule(static) reduction(| : err)
+        for (int64_t k = 0; k < (int64_t)na; ++k) {
             int32_t lab = s->act[k].clusterId;
-            if (lab < 0) { s->err |= 64; continue; }
+            if (lab < 0) { err |= 64; continue; }
             /* lower_bound over sweepCuts by clusterId */
             uint32_t lo = 0, hi = s->ncut;
             while (lo < hi) {
@@ -454,19 +486,27 @@ void ox_cut(ox_state* s) {
             if (lo == s->ncut || s->sccut[lo].clusterId != lab) continue;
             if (s->nd[k].offsetInCluster > s->sccut[lo].offset) {
                 int64_t upd = (int64_t)lo + (int64_t)maxLabel + 1;
-                if (upd > 2147483647) s->err |= 128;
+                if (upd > 2147483647) err |= 128;
                 s->act[k].clusterId = (int32_t)upd;
                 s->labels[s->act[k].nix] = (int32_t)upd;
             }
         }
     }
+    s->err |= err;
     /* K13-K17 ConsolidationManager::consolidate, cuda/devConsolidate.h:168-204:
-     * components of same-label valid arcs of active nodes; label = min index */
-    for (uint32_t k = 0; k < na; ++k) s->uf[s->act[k].nix] = s->act[k].nix;
+     * components of same-label valid arcs of active nodes; label = min index.
+     * Union-find that always hangs the larger root below the smaller: every
+     * tree's root is its minimum node whatever the order the arcs are taken
+     * in, so the active nodes are visited in index order (memory order). */
+    uint8_t* mark = (uint8_t*)calloc(s->n ? s->n : 1, 1);
     for (uint32_t k = 0; k < na; ++k) {
-        uint32_t u = s->act[k].nix;
-        int32_t lab = s->act[k].clusterId;
-        if (lab < 0) continue;
+        if (s->act[k].clusterId < 0) continue;
+        mark[s->act[k].nix] = 1;
+        s->uf[s->act[k].nix] = s->act[k].nix;
+    }
+    for (uint32_t u = 0; u < s->n; ++u) {
+        if (!mark[u]) continue;
+        int32_t lab = s->labels[u];
         uint32_t start = s->ranges[u], end = start + s->deg[u];
         for (uint32_t j = start; j < end; ++j) {
             uint32_t v = s->nbr[j];
@@ -475,15 +515,14 @@ void ox_cut(ox_state* s) {
             if (ru < rv) s->uf[rv] = ru; else if (rv < ru) s->uf[ru] = rv;
         }
     }
-    for (uint32_t k = 0; k < na; ++k) {
-        if (s->act[k].clusterId < 0) continue;
-        uint32_t r = uf_find(s->uf, s->act[k].nix);
-        s->act[k].clusterId = (int32_t)r;
-        s->labels[s->act[k].nix] = (int32_t)r;
-    }
+    for (uint32_t u = 0; u < s->n; ++u)
+        if (mark[u]) s->labels[u] = (int32_t)uf_find(s->uf, u);
+    free(mark);
+    for (uint32_t k = 0; k < na; ++k)
+        if (s->act[k].clusterId >= 0) s->act[k].clusterId = s->labels[s->act[k].nix];
     /* K19-K20 computeClusterData, cuda/devPartition.h:380-411 */
     for (uint32_t k = 0; k < na; ++k) s->idx[k] = k;
-    msort_lab(s->idx, s->tmp, 0, na, s->act);
+    rsort_lab(s->idx, s->tmp, na, s->act);
     ox_lnode* sorted = (ox_lnode*)malloc((size_t)(na ? na : 1) * sizeof(ox_lnode));
     for (uint32_t k = 0; k < na; ++k) sorted[k] = s->act[s->idx[k]];
     memcpy(s->act, sorted, (size_t)na * sizeof(ox_lnode));
@@ -530,9 +569,11 @@ void ox_cut(ox_state* s) {
         newlab[c] = surv;
         surv += keep;
     }
-    for (uint32_t k = 0; k < na; ++k) {
+    err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
+    for (int64_t k = 0; k < (int64_t)na; ++k) {
         int32_t lab = s->act[k].clusterId;
-        if (lab < 0) { s->err |= 64; continue; }
+        if (lab < 0) { err |= 64; continue; }
         int32_t c = s->lookup[lab];
         const ox_cdata* cd = &s->cdata[c];
         float pot = cd->maxPotential - cd->minPotential;
@@ -547,10 +588,13 @@ void ox_cut(ox_state* s) {
             xm[s->act[k].nix] -= avg;
         }
     }
+    s->err |= err;
     free(newlab);
     s->nac = (uint32_t)surv;
-    /* K23 disableEdgesKernel_node, cuda/devPartition.h:80-146, 503-522 */
-    for (uint32_t k = 0; k < na; ++k) {
+    /* K23 disableEdgesKernel_node, cuda/devPartition.h:80-146, 503-522: each
+     * position rewrites only its own node's row and internal degree */
+#pragma omp parallel for schedule(dynamic, 2048)
+    for (int64_t k = 0; k < (int64_t)na; ++k) {
         uint32_t u = s->act[k].nix;
         int32_t lab = s->act[k].clusterId;
         if (lab < 0) continue;
@@ -585,7 +629,7 @@ int ox_final_partition(ox_state* s, int32_t* ids_out) {
     /* sort ascending (insertion-free: use merge sort through idx on u32 keys) */
     uint64_t* k = (uint64_t*)calloc((size_t)(n ? n : 1), 8);
     for (uint32_t i = 0; i < n; ++i) { k[i] = (uint64_t)((uint32_t)u[i] ^ 0x80000000u); s->idx[i] = i; }
-    msort_u64(s->idx, s->tmp, 0, n, k);
+    rsort_u64(s->idx, s->tmp, n, k);
     int32_t* uniq = (int32_t*)malloc((size_t)(n ? n : 1) * 4);
     int nu = 0;
     for (uint32_t i = 0; i < n; ++i) {

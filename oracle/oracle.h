@@ -90,6 +90,10 @@ void ox_get_sweepcuts(ox_state* s, ox_sweepcut* out);   /* sweepCutsBuffer[0..C)
 void ox_get_cluster_labels(ox_state* s, int32_t* out);  /* clusterLabels[0..C) */
 void ox_get_cluster_data(ox_state* s, ox_cdata* out);   /* clusterData[0..C) */
 
+/* OpenMP threads used by the calling thread's later calls (results do not
+ * depend on it) */
+void ox_set_threads(int n);
+
 /* building blocks, exported for the oracle's own pinning tests */
 void   ox_init_vector(uint32_t n, const uint32_t* degrees, uint32_t seed, double* out);
 float  ox_init_value(uint32_t idx, uint32_t deg, uint32_t seed);

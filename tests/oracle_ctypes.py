@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
         L.ox_finalize.restype = C.c_uint32
         L.ox_verify_cut_edges.argtypes = [C.c_uint32, _u32p, _u32p, _i32p]
         L.ox_verify_cut_edges.restype = C.c_uint32
+        L.ox_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -154,6 +155,11 @@ class Oracle:
 
     def cluster_labels(self):
         return self._get("ox_get_cluster_labels", self.counts()[1], np.int32)
+
+
+def set_threads(n: int):
+    """OpenMP threads of the calling Python thread's later oracle calls."""
+    lib().ox_set_threads(int(n))
 
 
 def init_vector(degrees, seed):

@@ -348,3 +348,147 @@ def test_connected_components_min_index_labels():
     comps = [[0, 1, 2, 3], [4], [5, 6, 7], [8], [9, 10], [11]]
     for i, comp in enumerate(comps):
         assert set(lab[comp]) == {-i - 1}, (comp, lab[comp])
+
+
+# ------------------------------------------------------------------ recentre, deactivation labels, finalize
+
+def _recentre_case(g, iters, seed_values=None):
+    """Runs the oracle and, at every cut, checks the recentring of
+    cuda/devPartition.h:624-711 from quantities computed here: survivors'
+    values become x - (double)rwSum / (double)count with rwSum the cluster's
+    exact float sum (C1) over the members' float values before the cut
+    (_exact_cluster_sum, rational arithmetic), deactivated nodes keep x."""
+    ox = Oracle(g, init_values=seed_values) if seed_values is not None else Oracle(g)
+    checked = 0
+    for it in range(iters):
+        if ox.counts()[1] == 0:
+            break
+        ox.walk()
+        ox.sweep()
+        before = ox.values().copy()
+        ox.cut()
+        after, lab = ox.values(), ox.labels()
+        was_active = np.zeros(g.n, bool)
+        for u in ox.active()["nix"]:
+            was_active[u] = True
+        for L in np.unique(lab[lab >= 0]):
+            mem = np.nonzero(lab == L)[0]
+            rw = _exact_cluster_sum(before[mem].astype(np.float32))
+            avg = float(rw) / float(len(mem))
+            want = before[mem] - avg
+            assert np.array_equal(after[mem].view(np.uint64), want.view(np.uint64)), (it, L)
+            # the survivors' float sum is what was subtracted: the new sum is
+            # the rounding left by one float sum, far below the old one
+            assert abs(after[mem].sum()) <= 2.0 ** -20 * np.abs(before[mem]).sum() + 1e-300, (it, L)
+            checked += 1
+        gone = (lab < 0)
+        assert np.array_equal(after[gone].view(np.uint64), before[gone].view(np.uint64))
+    return checked
+
+
+@pytest.mark.parametrize("name", ["torus16", "sbm500", "path50"])
+def test_recentre_subtracts_double_average_of_the_float_cluster_sum(name):
+    g = _graphs()[name]
+    assert _recentre_case(g, 12) > 0
+
+
+def test_recentre_catches_a_float_average():
+    """The check above tells a float average from the double one: on these
+    clusters (float)rwSum / (float)count rounds differently for some cluster,
+    and a double average of an FP64 sum (not the float sum) differs too."""
+    g = _graphs()["sbm500"]
+    ox = Oracle(g)
+    ox.walk(); ox.sweep()
+    before = ox.values().copy()
+    ox.cut()
+    after, lab = ox.values(), ox.labels()
+    diff_float, diff_f64sum = 0, 0
+    for L in np.unique(lab[lab >= 0]):
+        mem = np.nonzero(lab == L)[0]
+        rw = _exact_cluster_sum(before[mem].astype(np.float32))
+        favg = float(np.float32(rw) / np.float32(len(mem)))
+        davg = float(before[mem].astype(np.float32).astype(np.float64).sum()) / len(mem)
+        diff_float += not np.array_equal(after[mem], before[mem] - favg)
+        diff_f64sum += not np.array_equal(after[mem], before[mem] - davg)
+    assert diff_float > 0 and diff_f64sum > 0
+
+
+def test_deactivation_labels_collide_across_iterations_and_merge():
+    """Q1 (cuda/devSweepCut.h:530-537, cuda/devPartition.h:686-698, 798-823),
+    worked out by hand: node 0 is isolated, {1,2,3} and {4,5,6} are triangles
+    whose non-constant walk component shrinks by 0.1 - 0.9/2 = -0.35 a step,
+    with initial spreads 1.5e-3 and 4e-3 around -10 and +10.
+      it 1: the sweep cuts after {1,2,3,0} (conductance 0, rightmost), the
+            components are {0}, {1,2,3}, {4,5,6}; {0} has one node and is
+            deactivated with smallest(0) - 0 - 1 = -1.
+      it 2: smallest = -1 (node 0 still first in the sort); spreads 1.8e-4
+            and 4.9e-4 stay above 1e-4: no deactivation.
+      it 3: smallest = 0 again; {1,2,3} (spread 6.4e-5, cluster 0) gets
+            0 - 0 - 1 = -1 as well: the labels collide.
+      it 4: smallest = -1; {4,5,6} (spread 6.0e-5) gets -1 - 0 - 1 = -2.
+    The loop ends with iterations printed as 5; finalizePartition ranks
+    -2 -> 0, -1 -> 1, so node 0 and {1,2,3} form one cluster: 2 clusters."""
+    g = px.from_edges(7, [1, 1, 2, 4, 4, 5], [2, 3, 3, 5, 6, 6])
+    x0 = np.array([0.0, -10 - 7.5e-4, -10.0, -10 + 7.5e-4, 10 - 2e-3, 10.0, 10 + 2e-3])
+    want_smallest = [0, -1, 0, -1]
+    want_labels = [[-1, 0, 0, 0, 1, 1, 1], [-1, 0, 0, 0, 1, 1, 1], [-1, -1, -1, -1, 0, 0, 0],
+                   [-1, -1, -1, -1, -2, -2, -2]]
+    ox = Oracle(g, init_values=x0)
+    for it in range(4):
+        ox.walk(); ox.sweep()
+        assert ox.smallest() == want_smallest[it], it
+        ox.cut()
+        assert list(ox.labels()) == want_labels[it], it
+    assert ox.counts()[1] == 0
+    ox = Oracle(g, init_values=x0)
+    assert ox.run() == 5
+    ids, k = ox.final_partition()
+    assert k == 2 and list(ids) == [1, 1, 1, 1, 0, 0, 0]
+
+
+def _finalize_brute(n, fx, tx):
+    """lib/core/graph.h:147-184 by enumeration: both arcs of every edge,
+    sorted by (from, to); ranges[u] = first arc of u."""
+    arcs = sorted([(a, b) for a, b in zip(fx, tx)] + [(b, a) for a, b in zip(fx, tx)])
+    ranges = [0] * (n + 1)
+    for u in range(n + 1):
+        ranges[u] = sum(1 for a, _ in arcs if a < u)
+    return np.array(ranges, np.uint32), np.array([b for _, b in arcs], np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_ox_finalize_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 40))
+    m = int(rng.integers(1, 120))
+    fx = rng.integers(0, n, m).astype(np.uint32)
+    tx = rng.integers(0, n, m).astype(np.uint32)
+    q = m // 8
+    fx[:q] = tx[:q]                                  # loops
+    fx[q:2 * q], tx[q:2 * q] = fx[2 * q:3 * q], tx[2 * q:3 * q]   # duplicates
+    r = np.zeros(n + 1, np.uint32)
+    e = np.zeros(2 * m, np.uint32)
+    assert lib().ox_finalize(n, m, fx, tx, r, e) == 2 * m
+    wr, we = _finalize_brute(n, fx.tolist(), tx.tolist())
+    assert np.array_equal(r, wr) and np.array_equal(e, we)
+    g = px.from_edges(n, fx, tx)                     # the product's host CSR build agrees
+    assert np.array_equal(g.ranges, wr) and np.array_equal(g.edges, we)
+
+
+def test_hash_inverts_with_the_published_inverse_constant():
+    """The hash of cuda/devRandomWalk.h:29-34 is a bijection whose inverse
+    multiplies by 0x119de1f3, the inverse of 0x45d9f3b modulo 2^32: running the
+    inverse on the oracle's output returns every input.  A wrong constant or
+    shift in the oracle breaks the round trip."""
+    inv = 0x119DE1F3
+    assert (0x45D9F3B * inv) % (1 << 32) == 1
+
+    def unhash(x):
+        x = (((x >> 16) ^ x) * inv) & 0xFFFFFFFF
+        x = (((x >> 16) ^ x) * inv) & 0xFFFFFFFF
+        return ((x >> 16) ^ x) & 0xFFFFFFFF
+
+    rnd = random.Random(9)
+    xs = [0, 1, 6582, 0xFFFFFFFF, 0x80000000] + [rnd.getrandbits(32) for _ in range(2000)]
+    assert all(unhash(lib().ox_hash(x)) == x for x in xs)
+    assert lib().ox_hash(0) == 0 and lib().ox_hash(1) != 1
