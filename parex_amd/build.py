@@ -49,7 +49,7 @@ def build_pxgen():
 def build_engine():
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     _run([hipcc, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", os.path.join(CSRC, "pxd_engine.hip"),
-          os.path.join(CSRC, "pxd_ingest.hip"), "-o", os.path.join(PKG, "libpxd.so")])
+          "-o", os.path.join(PKG, "libpxd.so")])
 
 
 def build_cli():

@@ -504,9 +504,9 @@ struct pxd_engine {
         serving = false;
         HIPC(hipSetDevice(dev));
         std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
-        if (n >= 0x7FFFFFFFu || 2 * m >= 0xFFFFFFFFull)
-            return set_err(PXD_ERR_ARG, "graph too large for 32-bit indices (%u nodes, %llu edges)", n,
-                           (unsigned long long)m);
+        if (n >= 0x7FFFFFFFu || 2 * m >= 0x7FFFFFFFull)
+            return set_err(PXD_ERR_ARG, "device CSR build takes below 2^31 nodes and 2^31 arcs (%u nodes, %llu edges)",
+                           n, (unsigned long long)m);
         if (m && (!fx || !tx)) return set_err(PXD_ERR_ARG, "null edge arrays");
         G = 1;
         h_gnode.assign({0u, n});
@@ -516,11 +516,26 @@ struct pxd_engine {
         A = (uint32_t)(2 * m);
         M_total = m;
         CHK(allocate());
-        char msg[256] = {0};
-        const int rc = pxd_ingest_csr(n, m, fx, tx, d_row, d_nbr, st, msg, sizeof msg);
-        if (rc) {
-            h_gnode.clear();
-            return set_err(rc == 1 ? PXD_ERR_ARG : PXD_ERR_HIP, "%s", msg);
+        // the edge list and the build's work area: engine buffers, back to the
+        // pool with the rest of this input
+        uint32_t *fxd = nullptr, *txd = nullptr, *work = nullptr;
+        CHK(alloc(&fxd, m));
+        CHK(alloc(&txd, m));
+        CHK(alloc(&work, ingest_work_words(n, 2 * m)));
+        if (m) {
+            HIPC(hipMemcpyAsync(fxd, fx, m * 4, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(txd, tx, m * 4, hipMemcpyHostToDevice, st));
+        }
+        HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
+        HIPC(ingest_csr_async(n, m, fxd, txd, d_row, d_nbr, ingest_carve(work, n, 2 * m), &d_cnt->admit_bad, st));
+        uint32_t bad = 0;
+        CHK(read_u32(&d_cnt->admit_bad, &bad));
+        if (bad) {
+            for (uint64_t i = 0; i < m; ++i)
+                if (fx[i] >= n || tx[i] >= n)
+                    return set_err(PXD_ERR_ARG, "edge %llu (%u, %u) has an end >= n = %u", (unsigned long long)i,
+                                   fx[i], tx[i], n);
+            return set_err(PXD_ERR_STATE, "edge check failed");
         }
         std::vector<uint32_t> row((size_t)N + 1);
         HIPC(hipMemcpyAsync(row.data(), d_row, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost, st));
@@ -1383,17 +1398,11 @@ struct pxd_engine {
         if (ing_n) return set_err(PXD_ERR_ARG, "edge admissions are already enabled");
         if (max_n == 0 || max_n > N || 2 * max_m > A || 2 * max_m > 0x7FFFFFFFull)
             return set_err(PXD_ERR_ARG, "edge-admission limits beyond the serving capacity or 2^31 arcs");
-        const size_t tb = pxd_ingest_tmp_bytes(max_n, 2 * max_m);
-        if (!tb) return set_err(PXD_ERR_HIP, "sizing the device sort failed");
         CHK(alloc(&d_ifx, max_m));
         CHK(alloc(&d_itx, max_m));
-        CHK(alloc(&ing.keys_in, 2 * max_m));
-        CHK(alloc(&ing.keys_out, 2 * max_m));
-        CHK(alloc(&ing.deg, (size_t)max_n + 1));
-        char* tmp = nullptr;
-        CHK(alloc(&tmp, tb));
-        ing.tmp = tmp;
-        ing.tmp_bytes = tb;
+        uint32_t* work = nullptr;
+        CHK(alloc(&work, ingest_work_words(max_n, 2 * max_m)));
+        ing = ingest_carve(work, max_n, 2 * max_m);
         ing_n = max_n;
         ing_m = max_m;
         return 0;
@@ -1431,7 +1440,7 @@ struct pxd_engine {
             HIPC(hipMemcpyAsync(d_ifx, fx, m * 4, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_itx, tx, m * 4, hipMemcpyHostToDevice, st));
         }
-        HIPC(pxd_ingest_csr_async(n, m, d_ifx, d_itx, d_stage + stage_used, d_nbr + ab, ing, d_pbad + k, st));
+        HIPC(ingest_csr_async(n, m, d_ifx, d_itx, d_stage + stage_used, d_nbr + ab, ing, d_pbad + k, st));
         pending.push_back(Pending{tag, (uint32_t)nb, n, (uint32_t)ab, (uint32_t)arcs, g, stage_used});
         stage_used += n + 1;
         live_graphs++;

@@ -66,6 +66,10 @@ def build(asan=False):
     src = open(os.path.join(CSRC, "pxd_engine.hip")).read()
     gen = os.path.join(OUT, "pxd_engine_emu.cpp")
     open(gen, "w").write(transform(src))
+    # headers with launches of their own: transformed copies next to the
+    # generated source, which its quoted includes find first
+    for h in ("pxd_ingest.h",):
+        open(os.path.join(OUT, h), "w").write(transform(open(os.path.join(CSRC, h)).read()))
     lib = os.path.join(OUT, "libpxd_emu_asan.so" if asan else "libpxd_emu.so")
     flags = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address"] if asan else ["-O2"]
     cmd = ["g++", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *flags, "-pthread",

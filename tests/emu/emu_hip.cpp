@@ -1,6 +1,5 @@
 // CPU emulation runtime (see emu_hip.h) -- test infrastructure only.
 #include "emu_hip.h"
-#include "pxd_ingest.h"
 
 #include <map>
 #include <memory>
@@ -107,14 +106,3 @@ float erfcinvf(float yf) {
     return (float)x;
 }
 
-// the device CSR build uses rocPRIM, which the emulator does not carry
-int pxd_ingest_csr(uint32_t, uint64_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, hipStream_t,
-                   char* err, size_t errlen) {
-    snprintf(err, errlen, "device CSR build not available in the CPU emulation");
-    return 2;
-}
-size_t pxd_ingest_tmp_bytes(uint32_t, uint64_t) { return 0; }
-hipError_t pxd_ingest_csr_async(uint32_t, uint64_t, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*,
-                                const IngestWork&, uint32_t*, hipStream_t) {
-    return 1;
-}

@@ -136,7 +136,7 @@ template <class T> inline T atomicMax(T* p, T v) {
 typedef int hipError_t;
 typedef void* hipStream_t;
 typedef struct emu_event* hipEvent_t;
-enum { hipSuccess = 0 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
 #define hipStreamNonBlocking 1
 

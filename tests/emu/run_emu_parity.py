@@ -32,6 +32,27 @@ GRAPHS = {
 }
 
 
+def _hub_edges(n):
+    fx = [0] * (n - 1) + list(range(1, n))
+    tx = list(range(1, n)) + [i % (n - 1) + 1 for i in range(2, n + 1)]
+    return n, np.array(fx, np.uint32), np.array(tx, np.uint32)
+
+
+def _loops_dups_edges():
+    rng = np.random.default_rng(5)
+    fx = rng.integers(0, 300, 2000).astype(np.uint32)
+    tx = rng.integers(0, 300, 2000).astype(np.uint32)
+    fx[:10] = tx[:10]
+    fx[10:20], tx[10:20] = fx[20:30], tx[20:30]
+    return 300, fx, tx
+
+
+# device CSR builds (pxd_initialize_edges) against the host build
+# (lib/core/graph.h:147-184): rows of a wave, and one row of 2999 arcs above
+# a workgroup's tile (the long-row digit passes)
+EDGE_LISTS = {"ingest_loops_dups": _loops_dups_edges, "ingest_hub3000": lambda: _hub_edges(3000)}
+
+
 def main(lib_path, names):
     L = C.CDLL(lib_path)
     L.pxd_last_error.restype = C.c_char_p
@@ -42,6 +63,22 @@ def main(lib_path, names):
 
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
     for name in names:
+        if name in EDGE_LISTS:
+            n, fx, tx = EDGE_LISTS[name]()
+            g = px.from_edges(n, fx, tx)
+            p = Params()
+            L.pxd_default_params(C.byref(p))
+            h = C.c_void_p()
+            chk(L.pxd_create(C.byref(p), C.byref(h)))
+            chk(L.pxd_initialize_edges(h, n, C.c_uint64(len(fx)), ptr(fx), ptr(tx)))
+            r, e = np.zeros(n + 1, np.uint32), np.zeros(2 * len(fx), np.uint32)
+            chk(L.pxd_download_graph(h, ptr(r), ptr(e)))
+            L.pxd_destroy(h)
+            same = np.array_equal(r, g.ranges) and np.array_equal(e, g.edges)
+            print(f"{name}: n={n} m={len(fx)} CSR {'equal' if same else 'DIFFERENT'}", flush=True)
+            if not same:
+                return 1
+            continue
         g = GRAPHS[name]()
         p = Params()
         L.pxd_default_params(C.byref(p))
@@ -71,4 +108,4 @@ def main(lib_path, names):
 
 
 if __name__ == "__main__":
-    sys.exit(main(sys.argv[1], sys.argv[2:] or list(GRAPHS)))
+    sys.exit(main(sys.argv[1], sys.argv[2:] or list(GRAPHS) + list(EDGE_LISTS)))
