@@ -219,6 +219,27 @@ int pxd_serve_take(pxd_engine* e, int32_t* ids_out, int32_t* num_clusters, int32
 int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, uint64_t* live_arcs,
                     uint32_t* active_nodes, uint32_t* waiting_results);
 
+/* ------------------------------------------------------------------ allocation-free CSR build
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the device CSR build of pxd_initialize_edges (the
+ * reference's readGraph + Graph::finalize, lib/utils/graph_io.h:49-84 and
+ * lib/core/graph.h:147-184) from a device-resident undirected edge list
+ * (d_fx[i], d_tx[i]), i < m, into caller-owned device buffers d_ranges[n+1]
+ * and d_edges[2m], with the same kernels.  The work area is a static device
+ * array of the library (about 25 MB per GPU, held until the process ends), so
+ * the call allocates nothing: n <= PXD_NOALLOC_MAX_NODES and m <=
+ * PXD_NOALLOC_MAX_EDGES, larger graphs are refused with PXD_ERR_ARG.
+ * Asynchronous: queued on `stream` (a hipStream_t; NULL = the null stream),
+ * nothing is waited for.  *d_bad (device memory) becomes nonzero when an
+ * end is >= n (the CSR is then meaningless).  Every call on one GPU shares
+ * that work area: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialized by the caller. */
+#define PXD_NOALLOC_MAX_NODES (1u << 20)
+#define PXD_NOALLOC_MAX_EDGES (1ull << 21)
+int pxd_csr_from_edges_noalloc(uint32_t n, uint64_t m, const uint32_t* d_fx, const uint32_t* d_tx,
+                               uint32_t* d_ranges, uint32_t* d_edges, uint32_t* d_bad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,6 +146,7 @@ def pxd() -> C.CDLL:
         lib.pxd_serve_wait.argtypes = [P]
         lib.pxd_serve_stats.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.pxd_csr_from_edges_noalloc.argtypes = [C.c_uint32, C.c_uint64, P, P, P, P, P, P]
         lib._typed = True
     return lib
 
@@ -162,8 +163,11 @@ PXD_SYMBOLS = [
     "pxd_set_walk_variant", "pxd_walk_tuning", "pxd_initialize_edges", "pxd_serve_enable_edges",
     "pxd_serve_admit_edges",
     "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
-    "pxd_serve_stats",
+    "pxd_serve_stats", "pxd_csr_from_edges_noalloc",
 ]
+
+NOALLOC_MAX_NODES = 1 << 20    # PXD_NOALLOC_MAX_NODES
+NOALLOC_MAX_EDGES = 1 << 21    # PXD_NOALLOC_MAX_EDGES
 
 PXD_FULL = 1       # pxd_serve_admit: no room right now
 PXD_EMPTY = 1      # pxd_serve_next / pxd_serve_take: no finished result waiting
@@ -380,6 +384,20 @@ class Server:
         self._check(self._lib.pxd_serve_stats(self._h, C.byref(g), C.byref(nn), C.byref(aa), C.byref(na), C.byref(w)))
         return {"live_graphs": g.value, "live_nodes": nn.value, "live_arcs": aa.value, "active_nodes": na.value,
                 "waiting_results": w.value}
+
+
+def csr_from_edges_noalloc(n: int, m: int, fx_ptr: int, tx_ptr: int, ranges_ptr: int, edges_ptr: int, bad_ptr: int,
+                           stream: int = 0):
+    """pxd_csr_from_edges_noalloc: the device CSR build of DeviceManager.initialize_edges
+    from device edge arrays (fx, tx: u32[m]) into device buffers ranges u32[n+1] and
+    edges u32[2m], with the library's static work area (no device allocation); queued
+    on the HIP stream `stream` (0 = null stream).  *bad (device u32) becomes nonzero
+    when an end is >= n.  Arguments are device addresses (ints)."""
+    lib = pxd()
+    rc = lib.pxd_csr_from_edges_noalloc(int(n), int(m), fx_ptr or None, tx_ptr or None, ranges_ptr or None,
+                                        edges_ptr or None, bad_ptr or None, stream or None)
+    if rc != 0:
+        raise PxdError(rc, lib.pxd_last_error().decode())
 
 
 class DeviceManager:

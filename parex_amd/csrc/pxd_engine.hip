@@ -1579,6 +1579,37 @@ void pxd_engine::k_mark_seen_launch(uint32_t n, const uint32_t* cgraph) {
     k_mark_seen<<<grid_for(n, BS), BS, 0, st>>>(n, cgraph, d_gseen);
 }
 
+// =================================================================== allocation-free CSR build
+
+// The work area of pxd_csr_from_edges_noalloc: a static device array, so the
+// call allocates nothing.  Every GPU the library runs on holds its own copy
+// (about 25 MB) until the process ends.  Host code never passes `scratch`
+// itself (on the host it only names the symbol): it uses the device address
+// that hipGetSymbolAddress gives for the current device.
+constexpr uint32_t NOALLOC_MAX_NODES = PXD_NOALLOC_MAX_NODES;
+constexpr uint64_t NOALLOC_MAX_EDGES = PXD_NOALLOC_MAX_EDGES;
+constexpr uint64_t NOALLOC_WORDS = ingest_work_words(NOALLOC_MAX_NODES, 2 * NOALLOC_MAX_EDGES);
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_dev[64];   // device address of `scratch`, per device ordinal
+
+// the current device's copy of `scratch`, looked up once per device
+static int scratch_on_current_device(uint32_t** out) {
+    int dev = -1;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err(PXD_ERR_ARG, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(PXD_ERR_HIP, "hipGetSymbolAddress returned no address");
+        g_scratch_dev[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -2044,6 +2075,27 @@ int pxd_serve_stats(pxd_engine* e, uint32_t* live_graphs, uint64_t* live_nodes, 
             *waiting_results = (uint32_t)e->done.size();
         }
         return PXD_OK;
+    });
+}
+
+int pxd_csr_from_edges_noalloc(uint32_t n, uint64_t m, const uint32_t* d_fx, const uint32_t* d_tx,
+                               uint32_t* d_ranges, uint32_t* d_edges, uint32_t* d_bad, void* stream) {
+    return guarded([&]() -> int {
+        if (n > NOALLOC_MAX_NODES || m > NOALLOC_MAX_EDGES)
+            return set_err(PXD_ERR_ARG,
+                           "pxd_csr_from_edges_noalloc takes at most %u nodes and %llu edges (%u nodes, %llu edges "
+                           "given)",
+                           NOALLOC_MAX_NODES, (unsigned long long)NOALLOC_MAX_EDGES, n, (unsigned long long)m);
+        if (!d_ranges || !d_bad || (m && (!d_fx || !d_tx || !d_edges))) return set_err(PXD_ERR_ARG, "null argument");
+        uint32_t* base = nullptr;
+        CHK(scratch_on_current_device(&base));
+        const hipStream_t st = (hipStream_t)stream;
+        // every word the build reads is written first (counters and degrees by
+        // fill kernels, not memset): the last call's contents do not matter
+        k_fill_u32<<<1, 64, 0, st>>>(1, d_bad, 0u);
+        HIPC(hipGetLastError());
+        HIPC(ingest_csr_async(n, m, d_fx, d_tx, d_ranges, d_edges, ingest_carve(base, n, 2 * m), d_bad, st));
+        return 0;
     });
 }
 

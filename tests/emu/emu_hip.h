@@ -26,6 +26,7 @@
 #define __launch_bounds__(...)
 #define __restrict__ __restrict
 #define __shared__ static
+#define __align__(n) alignas(n)
 
 struct emu_dim3 { uint32_t x = 0, y = 1, z = 1; };
 struct double2 { double x, y; };
@@ -151,6 +152,9 @@ inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipGetLastError() { return 0; }
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
 inline hipError_t hipSetDevice(int) { return 0; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return 0; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
