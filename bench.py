@@ -1,18 +1,23 @@
-"""Benchmark: expander decomposition on one MI355X, batched vs plain.
+"""Benchmark: expander decomposition on one MI355X, batched vs plain, and served.
 
 A step is one complete decomposition of the workload's graph set (every graph
-run to its final partition).  Inputs are resident in HBM when a step starts:
-`reset` (untimed) restores the engine from the device copy of the input.
+run to its final partition), timed as the reference's CLI times it
+(main.cpp:42-48): upload of the host CSR (pinned host buffers) + initial
+state + decomposition.  `value` is that rate; `value_resident` is the
+decomposition alone with the input already in HBM.
 
-Default workload (`--workload sbm512`): 512 independent planted
-partition graphs, n log-uniform in [1K, 256K], k ~ U{4..64}, intra degree
-~ U(8, 32), inter degree ~ U(0.5, 3), seed 11; parameters seed 6582,
-sweep threshold 0.3, walk threshold 1e-4.
+Default workload (`--workload sbm512`): 512 independent planted-partition
+graphs, n log-uniform in [1K, 256K], k ~ U{4..64}, intra degree ~ U(8, 32),
+inter degree ~ U(0.5, 3), seed 11; parameters seed 6582, sweep threshold
+0.3, walk threshold 1e-4.  The default run also serves a bounded prefix
+(--serve-duration seconds, default 20) of the serving trace: Poisson arrivals
+at 200 requests/s, 60 % SBM 10K-200K nodes, 30 % tori 64^2-1024^2, 10 % RMAT
+scale 14-21 (seed 13), reporting p50/p99 request latency and completed edges/s.
 
 Printed (one JSON line on stdout): batched edges/s as `value`, the plain
-one-graph-per-engine path, their ratio, the walk kernel's HBM roofline and a
-CPU baseline (the reference's single-threaded CPU decomposition, oracle/_ref,
-on a bounded sample of the same graphs).
+one-graph-at-a-time path, their ratio, the walk kernel's HBM roofline, a CPU
+baseline (the reference's single-threaded CPU decomposition, oracle/_ref, on a
+bounded sample of the same graphs) and the serving leg.
 """
 import argparse
 import json
@@ -22,6 +27,7 @@ import subprocess
 import sys
 import tempfile
 import time
+from collections import deque
 
 import numpy as np
 
@@ -61,6 +67,25 @@ def workload(name, graphs_limit=None):
     raise SystemExit(f"unknown workload {name}")
 
 
+def pinned_u32(a, keep):
+    """A copy of a u32 array in pinned host memory (uploads are plain DMA)."""
+    import torch
+    t = torch.empty(max(a.size, 1), dtype=torch.int32, pin_memory=True)
+    keep.append(t)
+    v = t.numpy().view(np.uint32)[:a.size]
+    v[:] = a
+    return v
+
+
+def pinned_graphs(graphs):
+    """The graphs' CSR arrays in pinned host memory, as a client's receive
+    buffers would hold them."""
+    keep = []
+    return [px.Graph(g.n, g.m, pinned_u32(g.ranges, keep), pinned_u32(g.edges, keep)) for g in graphs], keep
+
+
+# --------------------------------------------------------------------------- serving
+
 def serve_pool(seed=13):
     """Request graphs of the serving mix: SBM with 10K-200K nodes (n log-uniform,
     other parameters as the 512-graph batch), tori with side 64-1024, RMAT with
@@ -90,59 +115,40 @@ def serve_trace(pool, rate=200.0, dur=60.0, seed=13):
 
 
 def pinned_pool(pool):
-    """Copies of the request graphs in pinned host memory, as a server's
-    receive buffers would be: uploads are then plain DMA."""
-    import torch
     keep, out = [], {}
     for kind, graphs in pool.items():
-        out[kind] = []
-        for g in graphs:
-            arrs = []
-            for a in (g.ranges, g.edges):
-                t = torch.empty(max(a.size, 1), dtype=torch.int32, pin_memory=True)
-                keep.append(t)
-                v = t.numpy().view(np.uint32)[:a.size]
-                v[:] = a
-                arrs.append(v)
-            out[kind].append(px.Graph(g.n, g.m, arrs[0], arrs[1]))
+        out[kind] = [px.Graph(g.n, g.m, pinned_u32(g.ranges, keep), pinned_u32(g.edges, keep)) for g in graphs]
     return out, keep
 
 
 def edge_lists(pool):
     """Each pool graph as an undirected edge list (every edge once) in pinned
     host memory: requests of clients that send edges, not a CSR."""
-    import torch
     keep, out = [], {}
     for kind, graphs in pool.items():
         out[kind] = []
         for g in graphs:
             src = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(g.ranges))
             sel = src < g.edges
-            arrs = []
-            for a in (src[sel], g.edges[sel]):
-                t = torch.empty(max(a.size, 1), dtype=torch.int32, pin_memory=True)
-                keep.append(t)
-                v = t.numpy().view(np.uint32)[:a.size]
-                v[:] = a
-                arrs.append(v)
-            out[kind].append((g.n, arrs[0], arrs[1]))
+            out[kind].append((g.n, pinned_u32(src[sel], keep), pinned_u32(g.edges[sel], keep)))
     return out, keep
 
 
-def serve_workload(a):
+def run_serving(dur, walk_variant=None, admit_kind="csr", check=True):
+    """Serve the first `dur` seconds of the trace in real time.  Latency =
+    time the client holds the result - arrival.  Returns a dict."""
     t = time.perf_counter()
     pool, _pinned = pinned_pool(serve_pool())
-    epool, _epinned = edge_lists(pool) if a.admit == "edges" else (None, None)
+    epool, _epinned = edge_lists(pool) if admit_kind == "edges" else (None, None)
     log(f"serving pool: {sum(len(v) for v in pool.values())} graphs ({time.perf_counter() - t:.1f} s)")
-    dur = a.duration
     arr, picks, rng = serve_trace(pool, 200.0, dur, 13)
     req = [pool[k][i] for k, i in picks]
     offered = sum(g.m for g in req) / dur
     log(f"{len(req)} requests over {dur:.0f} s, offered {offered:.3e} edges/s")
     srv = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=4096,
                     params=px.Params(record_node_data=False))
-    if a.walk_variant is not None:
-        srv.set_walk_variant(a.walk_variant)
+    if walk_variant is not None:
+        srv.set_walk_variant(walk_variant)
     if epool is not None:
         # requests as edge lists: the CSR of each is built on the device at admission
         srv.enable_edges(max(g.n for v in pool.values() for g in v), max(g.m for v in pool.values() for g in v))
@@ -163,10 +169,8 @@ def serve_workload(a):
     while srv.stats()["live_graphs"]:
         srv.step()
     srv.results(wait=True)
-    from collections import deque
     queue, lat, done_edges = deque(), np.zeros(len(req)), 0
     finished, i, steps, inflight_sum = 0, 0, 0, 0
-    tsp = {"admit": 0.0, "step": 0.0, "results": 0.0, "stats": 0.0, "idle": 0.0}
     t0 = time.perf_counter()
     last_log = t0
     while finished < len(req):
@@ -174,28 +178,19 @@ def serve_workload(a):
         while i < len(req) and arr[i] <= now:
             queue.append(i)
             i += 1
-        ta = time.perf_counter()
         while queue and admit(queue[0], queue[0]):
             queue.popleft()
-        tb = time.perf_counter()
         live = srv.stats()["live_graphs"]
-        tc0 = time.perf_counter()
-        tsp["admit"] += tb - ta
-        tsp["stats"] += tc0 - tb
         if live:
             srv.step()
             steps += 1
             inflight_sum += live
-            tc = time.perf_counter() - t0
-            tsp["step"] += tc + t0 - tc0
             for tag, ids, k, it in srv.results():
                 lat[tag] = time.perf_counter() - t0 - arr[tag]     # when the client has the result
                 done_edges += req[tag].m
                 finished += 1
-            tsp["results"] += time.perf_counter() - t0 - tc
         elif i < len(req):
             time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
-            tsp["idle"] += time.perf_counter() - tc0
         elif not queue:               # nothing in flight: collect the last results
             for tag, ids, k, it in srv.results(wait=True):
                 lat[tag] = time.perf_counter() - t0 - arr[tag]
@@ -205,55 +200,67 @@ def serve_workload(a):
             last_log = time.perf_counter()
             log(f"  t={last_log - t0:.0f} s: {finished}/{len(req)} done, {live} in flight, {len(queue)} queued")
     wall = time.perf_counter() - t0
-    srv_tuning = srv.walk_tuning()
-    # results in the mix equal solo results (sample, untimed)
-    same = True
-    probe = rng.choice(len(req), size=min(12, len(req)), replace=False)
-    srv2 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=64,
-                     params=px.Params(record_node_data=False))
-    ref = {}
-    for j in probe:
-        srv2.admit(req[j], int(j))
-        while srv2.stats()["live_graphs"]:
-            srv2.step()
-        for tag, ids, k, it in srv2.results(wait=True):
-            ref[tag] = (ids, k, it)
-    srv2.close()
-    # replay the probes inside a fresh mix to compare (the timed run does not keep ids)
-    srv3 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=256,
-                     params=px.Params(record_node_data=False))
-    mixset = list(probe) + list(rng.choice(len(req), size=24, replace=False))
-    for j, tagj in enumerate(mixset):
-        srv3.admit(req[tagj], (j << 32) | int(tagj))
-        srv3.step()
-    while srv3.stats()["live_graphs"]:
-        srv3.step()
-    for tag, ids, k, it in srv3.results(wait=True):
-        tj = tag & 0xFFFFFFFF
-        if tj in ref and (tag >> 32) < len(probe):
-            rid, rk, rit = ref[tj]
-            same = same and rk == k and rit == it and np.array_equal(rid, ids)
-    srv3.close()
+    tuning = srv.walk_tuning()
     srv.close()
+    same = None
+    if check:
+        # sampled requests alone and inside another mix give the same result (untimed)
+        same = True
+        probe = rng.choice(len(req), size=min(12, len(req)), replace=False)
+        srv2 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=64,
+                         params=px.Params(record_node_data=False))
+        ref = {}
+        for j in probe:
+            srv2.admit(req[j], int(j))
+            while srv2.stats()["live_graphs"]:
+                srv2.step()
+            for tag, ids, k, it in srv2.results(wait=True):
+                ref[tag] = (ids, k, it)
+        srv2.close()
+        srv3 = px.Server(node_capacity=48 << 20, arc_capacity=(1 << 32) - 2, graph_slots=256,
+                         params=px.Params(record_node_data=False))
+        mixset = list(probe) + list(rng.choice(len(req), size=min(24, len(req)), replace=False))
+        for j, tagj in enumerate(mixset):
+            srv3.admit(req[tagj], (j << 32) | int(tagj))
+            srv3.step()
+        while srv3.stats()["live_graphs"]:
+            srv3.step()
+        for tag, ids, k, it in srv3.results(wait=True):
+            tj = tag & 0xFFFFFFFF
+            if tj in ref and (tag >> 32) < len(probe):
+                rid, rk, rit = ref[tj]
+                same = same and rk == k and rit == it and np.array_equal(rid, ids)
+        srv3.close()
     p50, p99 = np.percentile(lat * 1e3, [50, 99])
-    value = done_edges / wall
+    return {"edges_per_s": done_edges / wall, "p50_ms": float(p50), "p99_ms": float(p99), "requests": len(req),
+            "duration_s": dur, "offered_edges_per_s": offered, "wall_s": wall, "steps": steps,
+            "ms_per_step": wall * 1e3 / max(steps, 1), "mean_in_flight": inflight_sum / max(steps, 1),
+            "results_equal_solo": same, "admission": "edge lists, CSR built on the device"
+            if admit_kind == "edges" else "CSR", "walk_variant": walk_variant,
+            "tuning": dict(zip(("walk", "contrib", "tunings"), tuning))}
+
+
+SERVE_DESC = ("serving: Poisson 200 req/s, 60% SBM 10K-200K nodes, 30% torus 64^2-1024^2, 10% RMAT scale 14-21 "
+              "ef 16 (seed 13); requests drawn from a pool of 72 graphs")
+
+
+def serve_workload(a):
+    s = run_serving(a.duration, a.walk_variant, a.admit)
     line = {
-        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": steps, "warmup": 3,
-        "ms_per_step": wall * 1e3 / max(steps, 1), "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": s["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": s["steps"],
+        "warmup": 3, "ms_per_step": s["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "serving: Poisson 200 req/s, 60% SBM 10K-200K nodes, 30% torus 64^2-1024^2, "
-                               "10% RMAT scale 14-21 ef 16; requests drawn from a pool of 72 graphs",
-                   "admission": "edge lists, CSR built on the device" if a.admit == "edges" else "CSR",
-                   "requests": len(req), "duration_s": dur, "seed": 13, "offered_edges_per_s": offered},
-        "p50_ms": p50, "p99_ms": p99, "mean_in_flight": inflight_sum / max(steps, 1),
-        "wall_s": wall, "results_equal_solo": same,
-        "walk_variant": a.walk_variant, "tuning": dict(zip(("walk", "contrib", "tunings"), srv_tuning)),
-        "host_seconds": {k: round(v, 3) for k, v in tsp.items()},
+        "config": {"workload": SERVE_DESC, "admission": s["admission"], "requests": s["requests"],
+                   "duration_s": s["duration_s"], "seed": 13, "offered_edges_per_s": s["offered_edges_per_s"]},
+        "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "mean_in_flight": s["mean_in_flight"], "wall_s": s["wall_s"],
+        "results_equal_solo": s["results_equal_solo"], "walk_variant": s["walk_variant"], "tuning": s["tuning"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0
 
+
+# --------------------------------------------------------------------------- measurement helpers
 
 def profiled_traffic(workload):
     """HBM bytes per k_walk launch from the committed rocprofv3 summary of
@@ -292,7 +299,8 @@ def cpu_baseline(graphs, budget_s):
                               f"lib/algorithms/expander_decomposition.h with Config{{0.3,1e-4,1,1}}, "
                               f"{r['seconds']:.2f} s on 1 core"}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_ctypes import Oracle
+    from oracle_ctypes import Oracle, set_threads
+    set_threads(1)
     t0, e, k = time.perf_counter(), 0, 0
     for g in graphs:
         if time.perf_counter() - t0 > budget_s:
@@ -362,6 +370,65 @@ def hierarchy_workload(a, k=16):
     return 0
 
 
+# --------------------------------------------------------------------------- batched and plain
+
+def batched_leg(pinned, warmup, steps):
+    """One engine re-initialized every step from the pinned host CSR (the
+    engine keeps its device buffers and, for inputs of the same sizes, its
+    tuning tables); each step times upload + initial state + decomposition."""
+    dm = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
+    for w in range(warmup):
+        t = time.perf_counter()
+        dm.initialize_batch(pinned)
+        it = dm.expanderDecomposition()
+        log(f"warmup {w}: {(time.perf_counter() - t) * 1e3:.1f} ms with upload, iterations {it}")
+    step_ms, init_ms, dec_ms = [], [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        dm.initialize_batch(pinned)
+        t1 = time.perf_counter()
+        it = dm.expanderDecomposition()
+        t2 = time.perf_counter()
+        step_ms.append((t2 - t0) * 1e3)
+        init_ms.append((t1 - t0) * 1e3)
+        dec_ms.append((t2 - t1) * 1e3)
+        log(f"step {k}: {step_ms[-1]:.1f} ms = upload+init {init_ms[-1]:.1f} + decomposition {dec_ms[-1]:.1f} "
+            f"({dm.lastRunMs():.1f} ms device), iterations {it}")
+    ids, ks = dm.getFinalPartition()
+    its = dm.graphIterations()
+    # the walk kernel's roofline: one more decomposition with every walk launch timed
+    dm.initialize_batch(pinned)
+    dm.enableWalkTiming(True)
+    dm.expanderDecomposition()
+    walk = dm.walkStats()
+    dm.enableWalkTiming(False)
+    tuning = dm.walkTuning()
+    dm.close()
+    return {"step_ms": float(np.mean(step_ms)), "init_ms": float(np.mean(init_ms)), "dec_ms": float(np.mean(dec_ms)),
+            "ids": ids, "ks": ks, "its": its, "walk": walk, "tuning": tuning}
+
+
+def plain_leg(pinned, batch):
+    """One graph at a time on one engine (re-initialized per graph, degree
+    rule kernels, no tuning launches), each timed like a step: upload +
+    initial state + decomposition.  Also checks every partition equals the
+    batch's."""
+    dm = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
+    dm.setWalkVariant(-2)
+    tot, same, off = 0.0, True, 0
+    for i, g in enumerate(pinned):
+        t = time.perf_counter()
+        dm.initialize(g)
+        it1 = dm.expanderDecomposition()
+        tot += time.perf_counter() - t
+        ids1, k1 = dm.getFinalPartition()
+        if not (np.array_equal(ids1, batch["ids"][off:off + g.n]) and k1 == batch["ks"][i] and it1 == batch["its"][i]):
+            same = False
+        off += g.n
+    dm.close()
+    return tot, same
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,7 +439,10 @@ def main():
     ap.add_argument("--graphs", type=int, default=None, help="use only the first G graphs of the set")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="batch workloads: skip the serving leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--serve-duration", type=float, default=20.0,
+                    help="batch workloads: seconds of the serving trace in the serving leg")
     ap.add_argument("--duration", type=float, default=60.0, help="serving workload: seconds of arrivals")
     ap.add_argument("--walk-variant", type=int, default=None,
                     help="serving workload: pin the per-arc kernel variant (default: online tuning)")
@@ -392,38 +462,15 @@ def main():
     graphs, cfg = workload(a.workload, a.graphs)
     n_tot = sum(g.n for g in graphs)
     m_tot = sum(g.m for g in graphs)
-    cfg.update({"nodes": n_tot, "edges": m_tot, "params": {"seed": 6582, "sc_threshold": 0.3, "rw_threshold": 1e-4}})
+    cfg.update({"nodes": n_tot, "edges": m_tot, "params": {"seed": 6582, "sc_threshold": 0.3, "rw_threshold": 1e-4},
+                "timed": "upload from pinned host memory + initial state + decomposition, per step"})
+    pinned, _keep = pinned_graphs(graphs)
     log(f"workload {a.workload}: {len(graphs)} graphs, {n_tot} nodes, {m_tot} edges ({time.perf_counter() - t:.1f} s)")
 
-    # ---------------------------------------------------------------- batched
-    dm = px.DeviceManager(px.Params(keep_original=True, record_node_data=False))
-    t = time.perf_counter()
-    dm.initialize_batch(graphs)
-    init_s = time.perf_counter() - t
-    log(f"batched initialize {init_s:.2f} s")
-    for w in range(a.warmup):
-        dm.reset()
-        it = dm.expanderDecomposition()
-        log(f"warmup {w}: {dm.lastRunMs():.1f} ms, iterations {it}")
-    step_ms = []
-    for k in range(a.steps):
-        dm.reset()
-        t = time.perf_counter()
-        it = dm.expanderDecomposition()
-        step_ms.append((time.perf_counter() - t) * 1e3)
-        log(f"step {k}: {step_ms[-1]:.1f} ms wall, {dm.lastRunMs():.1f} ms device, iterations {it}")
-    ms = float(np.mean(step_ms))
-    value = m_tot / (ms / 1e3)
-    batch_ids, batch_k = dm.getFinalPartition()
-    batch_it = dm.graphIterations()
-
-    # ---------------------------------------------------------------- walk roofline
-    dm.reset()
-    dm.enableWalkTiming(True)
-    dm.expanderDecomposition()
-    launches, walk_ms, walk_bytes = dm.walkStats()
-    dm.enableWalkTiming(False)
-    tuning, ctuning, tunings = dm.walkTuning()
+    b = batched_leg(pinned, a.warmup, a.steps)
+    value = m_tot / (b["step_ms"] / 1e3)
+    launches, walk_ms, walk_bytes = b["walk"]
+    tuning, ctuning, tunings = b["tuning"]
     achieved = walk_bytes / (walk_ms / 1e3) / 1e9 if walk_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -432,53 +479,39 @@ def main():
                 "avg_launch_us": round(walk_ms * 1e3 / max(launches, 1), 2),
                 "alg_bytes_per_launch": round(walk_bytes / max(launches, 1)),
                 # walk kernel chosen per log2(active-list length) class by the
-                # online tuner (first decomposition run): 0..5 = 2..64 lanes
-                # per row, 6/7/8 = LDS window with 1/2/4 lanes per row, 9 = LDS window,
-                # 2 lanes, 1024-entry chunks, 10 = 7 with clamped walk slots, 11 = LDS
-                # window over node-range chunks
+                # online tuner: 0..5 = 2..64 lanes per row, 6/7/8 = LDS window
+                # with 1/2/4 lanes per row, 9 = LDS window, 2 lanes, 1024-entry
+                # chunks, 10 = 7 with clamped walk slots, 11 = LDS window over
+                # node-range chunks
                 "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
                 "tunings": tunings}
-    dm.close()
 
-    # ---------------------------------------------------------------- plain
-    plain = None
-    same = None
+    plain, same = None, None
     if not a.no_plain:
-        tot = 0.0
-        same = True
-        off = 0
-        for i, g in enumerate(graphs):
-            d1 = px.DeviceManager(px.Params(keep_original=False, record_node_data=False))
-            d1.initialize(g)
-            d1.setWalkVariant(-2)   # the degree rule: no tuning launches inside the plain timing
-            t = time.perf_counter()
-            try:
-                it1 = d1.expanderDecomposition()
-            except px.PxdError as err:
-                log(f"plain graph {i} (n={g.n}, m={g.m}, avg degree {2 * g.m / max(g.n, 1):.1f}) failed: {err}")
-                raise
-            tot += time.perf_counter() - t
-            ids1, k1 = d1.getFinalPartition()
-            if a.workload == "sbm512" or len(graphs) > 1:
-                if not (np.array_equal(ids1, batch_ids[off:off + g.n]) and k1 == batch_k[i] and it1 == batch_it[i]):
-                    same = False
-            d1.close()
-            off += g.n
+        tot, same = plain_leg(pinned, b)
         plain = m_tot / tot
         log(f"plain: {tot * 1e3:.1f} ms total, {plain:.3e} edges/s; batched == plain: {same}")
 
-    # ---------------------------------------------------------------- CPU baseline
     cpu = None if a.no_cpu else cpu_baseline(graphs, a.cpu_budget)
+
+    serve = None
+    if not a.no_serve:
+        serve = run_serving(a.serve_duration)
+        log(f"serving: {serve['edges_per_s']:.3e} edges/s, p50 {serve['p50_ms']:.0f} ms, p99 {serve['p99_ms']:.0f} ms")
 
     line = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "ms_per_step": b["step_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic", "config": cfg,
-        "initialize_s": round(init_s, 3),
-        "value_incl_upload": m_tot / (ms / 1e3 + init_s),
+        "upload_init_ms": round(b["init_ms"], 2), "decomposition_ms": round(b["dec_ms"], 2),
+        "value_resident": m_tot / (b["dec_ms"] / 1e3),
         "plain_value": plain, "batched_over_plain": (value / plain) if plain else None,
         "batched_equals_plain": same,
-        "iterations_max": int(batch_it.max()) if len(batch_it) else 0,
+        "iterations_max": int(b["its"].max()) if len(b["its"]) else 0,
+        "p50_ms": serve["p50_ms"] if serve else None, "p99_ms": serve["p99_ms"] if serve else None,
+        "served_edges_per_s": serve["edges_per_s"] if serve else None,
+        "serving": ({k: v for k, v in serve.items() if k not in ("p50_ms", "p99_ms", "edges_per_s")} |
+                    {"workload": SERVE_DESC}) if serve else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -137,7 +137,8 @@ struct pxd_engine {
     bool inited = false;
 
     // graph
-    uint32_t *d_row = nullptr, *d_deg = nullptr, *d_nbr = nullptr, *d_nbr0 = nullptr, *d_gnode = nullptr;
+    uint32_t *d_row = nullptr, *d_deg = nullptr, *d_nbr = nullptr, *d_nbr0 = nullptr, *d_gnode = nullptr,
+             *d_garc = nullptr;
     // per node
     uint32_t* d_dint = nullptr;
     int32_t* d_label = nullptr;
@@ -227,6 +228,7 @@ struct pxd_engine {
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     int tuned[2][33];
+    uint64_t tuned_for[3] = {~0ull, ~0ull, ~0ull};   // (graphs, nodes, arcs) the tables were tuned on
     uint64_t walk_tunings = 0;
     double avg_deg = 0.0;
     hipEvent_t tev0 = nullptr, tev1 = nullptr;
@@ -334,6 +336,7 @@ struct pxd_engine {
         CHK(alloc(&d_nbr, A));
         if (P.keep_original) CHK(alloc(&d_nbr0, A));
         CHK(alloc(&d_gnode, G + 1));
+        CHK(alloc(&d_garc, G + 1));
         CHK(alloc(&d_dint, N));
         CHK(alloc(&d_label, N));
         for (int b = 0; b < 2; ++b) {
@@ -429,7 +432,6 @@ struct pxd_engine {
         release();
         serving = false;
         HIPC(hipSetDevice(dev));
-        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);   // a new input: tune again
         G = g;
         h_gnode.assign(G + 1, 0);
         h_garc.assign(G + 1, 0);
@@ -449,47 +451,53 @@ struct pxd_engine {
                            (unsigned long long)nn, (unsigned long long)aa);
         h_gnode[G] = (uint32_t)nn;
         h_garc[G] = (uint32_t)aa;
+        // the online tuner's tables belong to the input's sizes: a new input of
+        // the same sizes (the same workload uploaded again) keeps them
+        if (tuned_for[0] != G || tuned_for[1] != nn || tuned_for[2] != aa) {
+            std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+            tuned_for[0] = G;
+            tuned_for[1] = nn;
+            tuned_for[2] = aa;
+        }
         N = (uint32_t)nn;
         A = (uint32_t)aa;
         M_total = aa / 2;
 
-        maxdeg = 0;
-        // row offsets of the whole batch on the host (O(N)); the arcs go up as
-        // they are and are checked and rebased to global ids on the device
-        std::vector<uint32_t> row(N + 1);
-        for (uint32_t i = 0; i < G; ++i) {
-            const uint32_t nb = h_gnode[i], ab = h_garc[i];
-            for (uint32_t u = 0; u < ncnt[i]; ++u) {
-                if (rptr[i][u + 1] < rptr[i][u]) return set_err(PXD_ERR_ARG, "graph %u: ranges decrease at %u", i, u);
-                row[nb + u] = ab + rptr[i][u];
-                maxdeg = std::max(maxdeg, rptr[i][u + 1] - rptr[i][u]);
-            }
-        }
-        row[N] = A;
-
         CHK(allocate());
-        const size_t n1 = (size_t)N + 1;
-
+        // Every graph's ranges (first n entries) and arcs go up as they are:
+        // host work is O(graphs).  One pass over the rows rebases them to the
+        // batch's arc offsets, checks that they do not decrease and finds the
+        // largest degree; one pass over the arcs checks each below its graph's
+        // n and rebases it to global node ids.  One sync reads both results.
         HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
-        HIPC(hipMemcpyAsync(d_row, row.data(), n1 * 4, hipMemcpyHostToDevice, st));
         for (uint32_t i = 0; i < G; ++i) {
+            if (ncnt[i]) HIPC(hipMemcpyAsync(d_row + h_gnode[i], rptr[i], (size_t)ncnt[i] * 4, hipMemcpyHostToDevice, st));
             const uint64_t arcs = 2ull * ecnt[i];
-            if (!arcs) continue;
-            HIPC(hipMemcpyAsync(d_nbr + h_garc[i], eptr[i], arcs * 4, hipMemcpyHostToDevice, st));
-            k_rebase_check<<<grid_for(arcs, BS), BS, 0, st>>>(arcs, d_nbr + h_garc[i], ncnt[i], h_gnode[i],
-                                                               &d_cnt->admit_bad);
-            HIPC(hipGetLastError());
+            if (arcs) HIPC(hipMemcpyAsync(d_nbr + h_garc[i], eptr[i], arcs * 4, hipMemcpyHostToDevice, st));
         }
         HIPC(hipMemcpyAsync(d_gnode, h_gnode.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
-        uint32_t bad = 0;
-        CHK(read_u32(&d_cnt->admit_bad, &bad));   // also: the host vectors go out of scope after the sync
-        if (bad) {
+        HIPC(hipMemcpyAsync(d_garc, h_garc.data(), (G + 1) * 4, hipMemcpyHostToDevice, st));
+        k_fill_u32<<<1, 64, 0, st>>>(1, d_row + N, A);
+        if (N) {
+            k_rebase_rows<<<grid_for(N, BS), BS, 0, st>>>(N, G, d_gnode, d_garc, d_row);
+            k_check_rows<<<grid_for(N, BS), BS, 0, st>>>(N, d_row, d_cnt);
+        }
+        if (A) k_rebase_arcs<<<grid_for(A, REBASE_CHUNK), BS, 0, st>>>(A, G, d_gnode, d_garc, d_nbr, &d_cnt->admit_bad);
+        HIPC(hipGetLastError());
+        Counters hc0{};
+        HIPC(hipMemcpyAsync(&hc0, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        CHK(sync());                  // also: the host vectors go out of scope after the sync
+        maxdeg = hc0.maxdeg;
+        if (hc0.admit_bad) {          // name the first bad range or arc
+            for (uint32_t i = 0; i < G; ++i)
+                for (uint32_t u = 0; u < ncnt[i]; ++u)
+                    if (rptr[i][u + 1] < rptr[i][u]) return set_err(PXD_ERR_ARG, "graph %u: ranges decrease at %u", i, u);
             for (uint32_t i = 0; i < G; ++i)
                 for (uint64_t j = 0; j < 2ull * ecnt[i]; ++j)
                     if (eptr[i][j] >= ncnt[i])
                         return set_err(PXD_ERR_ARG, "graph %u: arc %llu points to %u >= n", i,
                                        (unsigned long long)j, eptr[i][j]);
-            return set_err(PXD_ERR_STATE, "arc check failed");
+            return set_err(PXD_ERR_STATE, "input check failed");
         }
         HIPC(hipMemsetAsync(d_cnt, 0, sizeof(Counters), st));
         if (P.keep_original && A)
@@ -504,6 +512,7 @@ struct pxd_engine {
         serving = false;
         HIPC(hipSetDevice(dev));
         std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+        tuned_for[0] = tuned_for[1] = tuned_for[2] = ~0ull;
         if (n >= 0x7FFFFFFFu || 2 * m >= 0x7FFFFFFFull)
             return set_err(PXD_ERR_ARG, "device CSR build takes below 2^31 nodes and 2^31 arcs (%u nodes, %llu edges)",
                            n, (unsigned long long)m);
@@ -1345,6 +1354,7 @@ struct pxd_engine {
         heavy_valid = false;
         rmap_valid = false;
         std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+        tuned_for[0] = tuned_for[1] = tuned_for[2] = ~0ull;
         serving = true;
         inited = true;
         return sync();

@@ -35,7 +35,8 @@ struct Counters {
     uint32_t cclist_n;
     uint32_t link_changed;
     uint32_t admit_bad;
-    uint32_t pad[7];
+    uint32_t maxdeg;          // largest degree of an uploaded batch
+    uint32_t pad[6];
 };
 
 // ------------------------------------------------------------------ init
@@ -149,18 +150,67 @@ __global__ void k_rebase(uint64_t n, uint32_t* __restrict__ nbr, uint32_t base) 
         nbr[j] += base;
 }
 
-// arcs of one graph of a batch, uploaded as they are: check every arc below n,
-// then local -> global node ids
-__global__ void k_rebase_check(uint64_t arcs, uint32_t* __restrict__ nbr, uint32_t n, uint32_t base,
-                               uint32_t* __restrict__ bad) {
-    uint32_t b = 0u;
-    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < arcs; j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t v = nbr[j];
-        b |= v >= n ? 1u : 0u;
-        nbr[j] = v + base;
+// graph of position i in a batch: the last g with off[g] <= i (off ascending,
+// G + 1 entries)
+__device__ __forceinline__ uint32_t batch_graph(const uint32_t* __restrict__ off, uint32_t G, uint32_t i) {
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Row offsets of a batch uploaded graph by graph as local ranges: u of graph
+// g gets the graph's arc offset added
+__global__ void k_rebase_rows(uint32_t N, uint32_t G, const uint32_t* __restrict__ gnode,
+                              const uint32_t* __restrict__ garc, uint32_t* __restrict__ row) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x)
+        row[u] += garc[batch_graph(gnode, G, u)];
+}
+
+// Rebased offsets must not decrease (each graph's ranges run from 0 to its
+// 2m, checked on the host, so this is every graph's check); the largest row
+// length goes to cnt->maxdeg
+__global__ void k_check_rows(uint32_t N, const uint32_t* __restrict__ row, Counters* cnt) {
+    uint32_t b = 0u, mx = 0u;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
+        const uint32_t a = row[u], e = row[u + 1];
+        if (e < a) b = 1u;
+        else mx = max(mx, e - a);
     }
     b = group_or<WAVE>(b);
-    if (b && lane_id() == 0) atomicOr(bad, b);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    if (lane_id() == 0) {
+        if (b) atomicOr(&cnt->admit_bad, 1u);
+        if (mx) atomicMax(&cnt->maxdeg, mx);
+    }
+}
+
+// Arcs of a batch uploaded as they are: every arc checked below its graph's
+// n, then local -> global node ids.  A workgroup takes REBASE_CHUNK
+// consecutive arcs (coalesced, each thread every BS-th arc); the graph of its
+// first arc is searched once, and each thread moves forward through the
+// graphs as its arcs do.
+constexpr uint32_t REBASE_CHUNK = BS * 16;
+__global__ void __launch_bounds__(BS) k_rebase_arcs(uint32_t A, uint32_t G, const uint32_t* __restrict__ gnode,
+                                                    const uint32_t* __restrict__ garc, uint32_t* __restrict__ nbr,
+                                                    uint32_t* __restrict__ bad) {
+    uint32_t b = 0u;
+    for (uint32_t c0 = blockIdx.x * REBASE_CHUNK; c0 < A; c0 += gridDim.x * REBASE_CHUNK) {
+        uint32_t g = batch_graph(garc, G, c0);
+        const uint32_t c1 = min(A, c0 + REBASE_CHUNK);
+        for (uint32_t j = c0 + threadIdx.x; j < c1; j += BS) {
+            while (j >= garc[g + 1]) ++g;
+            const uint32_t v = nbr[j];
+            const uint32_t n = gnode[g + 1] - gnode[g];
+            b |= v >= n ? 1u : 0u;
+            nbr[j] = v + gnode[g];
+        }
+    }
+    b = group_or<WAVE>(b);
+    if (b && lane_id() == 0) atomicOr(bad, 1u);
 }
 
 // the admitted graph's single cluster, appended as cluster c at positions
