@@ -170,7 +170,7 @@ struct pxd_engine {
     uint32_t *d_tA = nullptr, *d_tB = nullptr, *d_tC = nullptr;
     uint32_t* d_cclist = nullptr;
     uint32_t *d_lwave = nullptr, *d_lblock = nullptr, *d_llarge = nullptr;
-    uint32_t *d_lstart = nullptr, *d_llen = nullptr;
+    uint32_t *d_ltiles = nullptr, *d_ltoff = nullptr;   // tiles per large segment, their scan
     LargeSeg* d_ls = nullptr;
     uint32_t *d_tseg = nullptr, *d_tidx = nullptr, *d_hist = nullptr, *d_hscan = nullptr;
     uint32_t tiles_cap = 0, ls_cap = 0;
@@ -196,6 +196,8 @@ struct pxd_engine {
     // host mirrors
     int cur = 0, acur = 0, pcur = 0;
     uint32_t na = 0, nc = 0, ncut = 0, ncc = 0, nalist = 0;
+    uint32_t ncc_members = 0;         // positions of the clusters the sweep flagged for CC
+    bool giter_read = false;          // the cut's end-of-iteration read brought h_giter (serving)
     uint32_t iter1 = 0;
     bool sweep_done = false;
     int gs = 16;
@@ -391,8 +393,8 @@ struct pxd_engine {
         CHK(alloc(&d_llarge, n1));
         ls_cap = N / TILE + 2;
         tiles_cap = 2 * (N / TILE) + 4;
-        CHK(alloc(&d_lstart, ls_cap));
-        CHK(alloc(&d_llen, ls_cap));
+        CHK(alloc(&d_ltiles, ls_cap));
+        CHK(alloc(&d_ltoff, (size_t)ls_cap + 1));
         CHK(alloc(&d_ls, ls_cap));
         CHK(alloc(&d_tseg, tiles_cap));
         CHK(alloc(&d_tidx, tiles_cap));
@@ -634,105 +636,85 @@ struct pxd_engine {
     }
 
     // ---------------------------------------------------------------- scan
-    int scan_level(const uint32_t* in, uint32_t* out, uint32_t n, size_t lvl) {
+    int scan_level(const uint32_t* in, uint32_t* out, uint32_t n, size_t lvl, const uint32_t* pn, uint32_t* total) {
         const uint32_t nb = cdiv(n, BS * SCAN_ITEMS);
         if (nb <= 1) {
-            k_scan_down<<<1, BS, 0, st>>>(n, in, out, nullptr);
+            k_scan_down<<<1, BS, 0, st>>>(n, in, out, nullptr, pn, total);
             HIPC(hipGetLastError());
             return 0;
         }
         if (lvl >= lvl_sum.size()) return set_err(PXD_ERR_STATE, "scan too deep");
-        k_scan_up<<<nb, BS, 0, st>>>(n, in, lvl_sum[lvl]);
+        // the block sums cover the bound n (blocks past the device count add 0)
+        k_scan_up<<<nb, BS, 0, st>>>(n, in, lvl_sum[lvl], pn);
         HIPC(hipGetLastError());
-        CHK(scan_level(lvl_sum[lvl], lvl_scan[lvl], nb, lvl + 1));
-        k_scan_down<<<nb, BS, 0, st>>>(n, in, out, lvl_scan[lvl]);
+        CHK(scan_level(lvl_sum[lvl], lvl_scan[lvl], nb, lvl + 1, nullptr, nullptr));
+        k_scan_down<<<nb, BS, 0, st>>>(n, in, out, lvl_scan[lvl], pn, total);
         HIPC(hipGetLastError());
         return 0;
     }
-    // out[0..n) exclusive prefix sums, out[n] = total
-    int scan(const uint32_t* in, uint32_t* out, uint32_t n) {
+    // out[0..n) exclusive prefix sums, out[n] = total.  With pn the length is
+    // *pn, read on the device, and n is a bound on it that sizes the grids;
+    // total (when given) receives the sum too, at an address the host knows.
+    int scan(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_t* pn = nullptr, uint32_t* total = nullptr) {
         if (n == 0) {
             k_fill_u32<<<1, 64, 0, st>>>(1, out, 0u);
+            if (total) k_fill_u32<<<1, 64, 0, st>>>(1, total, 0u);
             HIPC(hipGetLastError());
             return 0;
         }
         if (n > BS * SCAN_ITEMS && n <= SMALL_SCAN_N) {
-            k_scan_small<<<1, 1024, 0, st>>>(n, in, out);
+            k_scan_small<<<1, 1024, 0, st>>>(n, in, out, pn, total);
             HIPC(hipGetLastError());
             return 0;
         }
-        return scan_level(in, out, n, 0);
+        return scan_level(in, out, n, 0, pn, total);
     }
 
     // ---------------------------------------------------------------- segmented sort
-    // Stable sort of (pkey[0], pnix[0]) inside every segment [segs[c], segs[c+1]).
-    int seg_sort(const uint32_t* segs, uint32_t nseg) {
-        if (nseg == 0) return 0;
+    // Stable sort of (pkey[0], pnix[0]) inside every segment [segs[c], segs[c+1])
+    // of the n items.  The size classes and the large segments' tile plan stay
+    // on the device (no host read): grids are sized from what n and nseg bound,
+    // and every kernel reads its list length from the counters.
+    int seg_sort(const uint32_t* segs, uint32_t nseg, uint32_t n) {
+        if (nseg == 0 || n < 2) return 0;
         HIPC(hipMemsetAsync(&d_cnt->nwave, 0, 3 * sizeof(uint32_t), st));
         k_classify<<<grid_for(nseg, BS), BS, 0, st>>>(nseg, segs, d_lwave, d_lblock, d_llarge, d_cnt);
         HIPC(hipGetLastError());
-        CHK(read_counters());
-        const uint32_t nwave = hc.nwave, nblock = hc.nblock, nlarge = hc.nlarge;
+        // class sizes: 2..64, 65..TILE and more than TILE items per segment
+        const uint32_t nwave = std::min(nseg, n / 2u), nblock = std::min(nseg, n / (uint32_t)(WAVE + 1)),
+                       nlarge = std::min(nseg, n / (uint32_t)(TILE + 1));
         if (nwave) {
-            k_sort_wave<<<cdiv(nwave, BS / WAVE), BS, 0, st>>>(nwave, d_lwave, segs, d_pkey[0], d_pnix[0]);
+            k_sort_wave<<<std::min(cdiv(nwave, BS / WAVE), 8192u), BS, 0, st>>>(&d_cnt->nwave, d_lwave, segs,
+                                                                                d_pkey[0], d_pnix[0]);
             HIPC(hipGetLastError());
         }
         if (nblock) {
-            k_sort_block<<<nblock, BS, 0, st>>>(nblock, d_lblock, segs, d_pkey[0], d_pnix[0]);
+            k_sort_block<<<std::min(nblock, 2048u), BS, 0, st>>>(&d_cnt->nblock, d_lblock, segs, d_pkey[0], d_pnix[0]);
             HIPC(hipGetLastError());
         }
         if (nlarge) {
-            if (nlarge > ls_cap) return set_err(PXD_ERR_STATE, "too many large segments");
-            // start and length of each large segment
-            k_gather<<<grid_for(nlarge, BS), BS, 0, st>>>(nlarge, d_llarge, segs, d_lstart);
+            // a large segment of len items has ceil(len / TILE) <= len / TILE + 1 tiles
+            const uint32_t tiles = n / (uint32_t)TILE + nlarge;
+            if (nlarge > ls_cap || tiles > tiles_cap) return set_err(PXD_ERR_STATE, "tile capacity exceeded");
+            k_large_plan<<<grid_for(nlarge, BS), BS, 0, st>>>(&d_cnt->nlarge, d_llarge, segs, d_ls, d_ltiles);
             HIPC(hipGetLastError());
-            std::vector<uint32_t> lst(nlarge), lid(nlarge);
-            HIPC(hipMemcpyAsync(lst.data(), d_lstart, nlarge * 4, hipMemcpyDeviceToHost, st));
-            HIPC(hipMemcpyAsync(lid.data(), d_llarge, nlarge * 4, hipMemcpyDeviceToHost, st));
-            // lengths need segs[c+1]: gather from segs + 1
-            k_gather<<<grid_for(nlarge, BS), BS, 0, st>>>(nlarge, d_llarge, segs + 1, d_llen);
+            CHK(scan(d_ltiles, d_ltoff, nlarge, &d_cnt->nlarge, &d_cnt->tiles));
+            k_large_expand<<<grid_for(nlarge, BS), BS, 0, st>>>(&d_cnt->nlarge, d_ltoff, d_ls, d_tseg, d_tidx, d_cnt);
             HIPC(hipGetLastError());
-            std::vector<uint32_t> lend(nlarge);
-            HIPC(hipMemcpyAsync(lend.data(), d_llen, nlarge * 4, hipMemcpyDeviceToHost, st));
-            CHK(sync());
-            // the plan lives in members: the copies below read it after this
-            // function returns, and the next plan is only written after the
-            // next call's classification sync, when these copies are done
-            std::vector<LargeSeg>& ls = h_ls;
-            std::vector<uint32_t>& tseg = h_tseg;
-            std::vector<uint32_t>& tidx = h_tidx;
-            ls.assign(nlarge, LargeSeg{});
-            tseg.clear();
-            tidx.clear();
-            uint32_t tiles = 0;
-            for (uint32_t i = 0; i < nlarge; ++i) {
-                ls[i].start = lst[i];
-                ls[i].len = lend[i] - lst[i];
-                ls[i].ntiles = cdiv(ls[i].len, TILE);
-                ls[i].hbase = tiles * 256u;
-                for (uint32_t t = 0; t < ls[i].ntiles; ++t) { tseg.push_back(i); tidx.push_back(t); }
-                tiles += ls[i].ntiles;
-            }
-            if (tiles > tiles_cap) return set_err(PXD_ERR_STATE, "tile capacity exceeded");
-            HIPC(hipMemcpyAsync(d_ls, ls.data(), nlarge * sizeof(LargeSeg), hipMemcpyHostToDevice, st));
-            HIPC(hipMemcpyAsync(d_tseg, tseg.data(), tiles * 4, hipMemcpyHostToDevice, st));
-            HIPC(hipMemcpyAsync(d_tidx, tidx.data(), tiles * 4, hipMemcpyHostToDevice, st));
             int src = 0;
-            for (int pass = 0; pass < 4; ++pass) {
+            for (int pass = 0; pass < 4; ++pass) {   // four passes: the result is back in buffer 0
                 const int sh = 8 * pass;
-                k_tile_hist<<<tiles, BS, 0, st>>>(tiles, d_tseg, d_tidx, d_ls, d_pkey[src], sh, d_hist);
+                k_tile_hist<<<tiles, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_pkey[src], sh, d_hist);
                 HIPC(hipGetLastError());
-                CHK(scan(d_hist, d_hscan, tiles * 256u));
-                k_tile_scatter<<<tiles, BS, 0, st>>>(tiles, d_tseg, d_tidx, d_ls, d_hscan, d_pkey[src], d_pnix[src], sh,
-                                                     d_pkey[src ^ 1], d_pnix[src ^ 1]);
+                CHK(scan(d_hist, d_hscan, tiles * 256u, &d_cnt->tiles256));
+                k_tile_scatter<<<tiles, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_hscan, d_pkey[src],
+                                                     d_pnix[src], sh, d_pkey[src ^ 1], d_pnix[src ^ 1]);
                 HIPC(hipGetLastError());
                 src ^= 1;
             }
         }
         return 0;
     }
-    std::vector<LargeSeg> h_ls;
-    std::vector<uint32_t> h_tseg, h_tidx;
 
     // ---------------------------------------------------------------- steps
     template <int GSZ>
@@ -866,9 +848,9 @@ struct pxd_engine {
         }
     }
     template <int GSZ>
-    void launch_link(const uint32_t* list, uint32_t nl) {
+    void launch_link(const uint32_t* list, uint32_t nl, const uint32_t* pn) {
         k_cc_link<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, list, d_row, d_dint, d_nbr, d_part, d_root,
-                                                                heavy_thr, d_cnt);
+                                                                heavy_thr, d_cnt, pn, &d_cnt->cc_done);
     }
     void launch_disable_win(int lanes, uint32_t nl) {
         constexpr uint32_t CHUNK = 4096;
@@ -958,11 +940,12 @@ struct pxd_engine {
         ClusterTable& T = ct[ctc];
         ncut = 0;
         ncc = 0;
+        ncc_members = 0;
         if (na == 0 || nc == 0) { sweep_done = true; return 0; }
         // keys in position order, then stable sort inside every cluster
         k_gather<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
         HIPC(hipGetLastError());
-        CHK(seg_sort(T.cs, nc));
+        CHK(seg_sort(T.cs, nc, na));
         CHK(heavy_map());
         launch_contrib_variant(pick_variant(1, [this](int v) { launch_contrib_variant(v); }));
         HIPC(hipGetLastError());
@@ -982,65 +965,78 @@ struct pxd_engine {
                                      d_cbest, P.record_node_data ? d_nd : nullptr);
         HIPC(hipGetLastError());
         CHK(zero_counter_fields());
-        k_decide<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cbest, (double)P.sc_threshold, T.cflag, d_coff, d_csp, d_cnt);
+        k_decide<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cbest, (double)P.sc_threshold, T.cflag, d_coff, d_csp, T.cs,
+                                                   d_cnt);
         HIPC(hipGetLastError());
         CHK(read_counters());
         ncut = hc.ncut;
         ncc = hc.ncc;
+        ncc_members = hc.ccn;
         sweep_done = true;
         return 0;
     }
 
+    // One host read per cut: after the compaction of positions and clusters
+    // (kept positions and clusters; in serving also every graph's iteration
+    // count).  Until then the cluster count after a CC regroup is known on the
+    // device only: the kernels read it from the counters and their grids are
+    // sized from the bound nc + (CC members).
     int cut() {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (!sweep_done) return set_err(PXD_ERR_ARG, "cutClusters needs a preceding computeSweepCuts");
         sweep_done = false;
         heavy_valid = false;          // the cut changes the active list and internal degrees
         rmap_valid = false;
+        giter_read = false;
         ++iter1;
         if (na == 0 || nc == 0) return 0;
         ClusterTable* T = &ct[ctc];
         uint32_t nl = 0;
+        uint32_t ncb = nc;                    // cluster count, or a bound on it
+        const uint32_t* pnc = nullptr;        // the device count when ncb is a bound
         if (ncc > 0) {
+            nl = ncc_members;
             k_mark<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
                                                     d_root, d_stamp);
-            // the CC list in node order (the active list holds the positions' nodes)
+            // the CC list in node order (the active list holds the positions' nodes);
+            // its length is the member count the sweep returned
             k_member_flags<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_stamp, iter1, d_tA);
             HIPC(hipGetLastError());
             CHK(scan(d_tA, d_tB, nalist));
             k_compact_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tA, d_tB, d_alist[acur], d_cclist);
             HIPC(hipGetLastError());
-            CHK(read_u32(d_tB + nalist, &nl));
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
+            k_one_segment<<<1, 64, 0, st>>>(d_oneseg, na, nullptr);
             HIPC(hipGetLastError());
-            const uint32_t one[2] = {0u, na};
-            HIPC(hipMemcpyAsync(d_oneseg, one, 8, hipMemcpyHostToDevice, st));
-            CHK(seg_sort(d_oneseg, 1));
+            CHK(seg_sort(d_oneseg, 1, na));
             k_run_flags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pkey[0], d_tA);
             HIPC(hipGetLastError());
-            CHK(scan(d_tA, d_tB, na));
+            CHK(scan(d_tA, d_tB, na, nullptr, &d_cnt->ncr));
             k_build_clusters<<<grid_for(na, BS), BS, 0, st>>>(na, d_ngraph, d_pkey[0], d_tA, d_tB, T->cs, T->cmin,
                                                               T->cgraph, T->cflag, d_pclu[0]);
             HIPC(hipGetLastError());
-            CHK(read_u32(d_tB + na, &nc));
+            // components of the CC clusters are at most their members
+            ncb = (uint32_t)std::min<uint64_t>(na, (uint64_t)nc + nl);
+            pnc = &d_cnt->ncr;
         }
         // statistics of the (possibly new) clusters
-        k_graph_first<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst);
-        k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cmaxo, d_cmino, d_csum, nullptr);
+        const uint32_t gc = grid_for(ncb, BS);
+        k_graph_first<<<gc, BS, 0, st>>>(ncb, T->cgraph, d_gfirst, pnc);
+        k_stats_reset<<<gc, BS, 0, st>>>(ncb, d_cmaxo, d_cmino, d_csum, nullptr, pnc);
         // positions of every cluster are in ascending walk-key order (sweep sort,
         // then the stable regroup): min and max are the first and last keys
-        k_stats_ends<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_pnix[0], d_xkey, d_cmaxo, d_cmino);
-        k_stats_scale<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK);
+        k_stats_ends<<<gc, BS, 0, st>>>(ncb, T->cs, d_pnix[0], d_xkey, d_cmaxo, d_cmino, pnc);
+        k_stats_scale<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, pnc);
         k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
-        k_stats_final<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold,
-                                                       d_ckeep, d_cavg);
+        k_stats_final<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold, d_ckeep, d_cavg,
+                                         pnc);
         HIPC(hipGetLastError());
-        CHK(scan(d_ckeep, d_kscan, nc));
-        k_labels<<<grid_for(nc, BS), BS, 0, st>>>(nc, T->cgraph, d_gfirst, d_ckeep, d_kscan, d_gsmall, d_gsmall_next,
-                                                  d_gsurv, T->clabel);
-        k_mark_seen_launch(nc, T->cgraph);
+        CHK(scan(d_ckeep, d_kscan, ncb, pnc, &d_cnt->nc_new));
+        k_labels<<<gc, BS, 0, st>>>(ncb, T->cgraph, d_gfirst, d_ckeep, d_kscan, d_gsmall, d_gsmall_next, d_gsurv,
+                                    T->clabel, pnc);
+        k_mark_seen_launch(ncb, T->cgraph, pnc);
         k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter,
                                                        d_gstart, iter1);
         // the active node list holds the same nodes as the positions: apply in index order
@@ -1050,17 +1046,18 @@ struct pxd_engine {
         HIPC(hipGetLastError());
         // drop deactivated clusters from the position list, the cluster table and the node list
         k_pos_keepflags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pclu[0], d_ckeep, d_tA);
-        CHK(scan(d_tA, d_tB, na));
+        CHK(scan(d_tA, d_tB, na, nullptr, &d_cnt->na_new));
         k_pos_compact<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_pclu[0], d_kscan, d_pnix[1],
                                                        d_pclu[1]);
-        HIPC(hipGetLastError());
-        uint32_t na_new = 0, nc_new = 0;
-        CHK(read_u32x2(d_tB + na, &na_new, d_kscan + nc, &nc_new));
         ClusterTable& T2 = ct[ctc ^ 1];
-        if (nc_new) {
-            k_clu_compact<<<grid_for(nc, BS), BS, 0, st>>>(nc, na_new, d_ckeep, d_kscan, d_tB, T->cols(), T2.cols());
-            HIPC(hipGetLastError());
+        k_clu_compact<<<gc, BS, 0, st>>>(ncb, 0u, d_ckeep, d_kscan, d_tB, T->cols(), T2.cols(), pnc, &d_cnt->na_new);
+        HIPC(hipGetLastError());
+        if (serving && G) {
+            HIPC(hipMemcpyAsync(h_giter.data(), d_giter, G * 4, hipMemcpyDeviceToHost, st));
+            giter_read = true;
         }
+        CHK(read_counters());
+        const uint32_t na_new = hc.na_new, nc_new = hc.nc_new;
         std::swap(d_pnix[0], d_pnix[1]);
         std::swap(d_pclu[0], d_pclu[1]);
         ctc ^= 1;
@@ -1102,13 +1099,16 @@ struct pxd_engine {
         return 0;
     }
 
-    void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph);
+    void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph, const uint32_t* pn = nullptr);
 
     // Components of every CC part (clusters flagged CF_CC in table T):
     // sampling rounds over the first arcs of each node, each followed by one
     // pointer-jumping launch (trees need not end flat: roots are found by
     // following pointers), then exact linking rounds over the members outside
     // their part's giant tree.  The regroup pass (k_rid) leaves every tree flat.
+    // The off-giant count stays on the device; the linking rounds run in
+    // batches, each round ended by k_cc_round, which stops the later rounds
+    // of a batch once one changed nothing: one host read per batch.
     int connected_components(uint32_t nl, const ClusterTable& T) {
         if (nl == 0) return 0;
         // the LDS window when the walk's choice for this list length is a window
@@ -1133,35 +1133,39 @@ struct pxd_engine {
         k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC);
         HIPC(hipGetLastError());
         CHK(scan(d_tC, d_tA, nl));
-        uint32_t nu = 0;
-        CHK(read_u32(d_tA + nl, &nu));
-        if (nu) {
-            k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);
+        const uint32_t* pnu = d_tA + nl;      // members outside the giant trees (<= nl)
+        k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);   // over all nl flags
+        HIPC(hipGetLastError());
+        const bool heavy = heavy_thr != 0xFFFFFFFFu;
+        if (heavy) {                  // chunk map of the list's long rows (the walk's map is not needed now)
+            heavy_valid = false;
+            rmap_valid = false;
+            k_heavy_count<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist2, d_dint, heavy_thr, d_hcnt, pnu);
+            CHK(scan(d_hcnt, d_hoff, nl, pnu, &d_cnt->hchunks));
+            k_heavy_expand<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_hcnt, d_hoff, d_hrow, pnu);
             HIPC(hipGetLastError());
-            const bool heavy = heavy_thr != 0xFFFFFFFFu;
-            if (heavy) {              // chunk map of the list's long rows (the walk's map is not needed now)
-                heavy_valid = false;
-                rmap_valid = false;
-                k_heavy_count<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_cclist2, d_dint, heavy_thr, d_hcnt);
-                CHK(scan(d_hcnt, d_hoff, nu));
-                k_heavy_expand<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_hcnt, d_hoff, d_hrow);
-                HIPC(hipGetLastError());
-            }
-            for (int round = 0;; ++round) {
-                HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
-                PXD_DISPATCH_GS(launch_link, d_cclist2, nu);
+        }
+        HIPC(hipMemsetAsync(&d_cnt->cc_done, 0, 4, st));
+        HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
+        uint32_t r = 0;
+        for (uint32_t batch = 3;; batch = std::min(2 * batch, 24u)) {
+            for (uint32_t b = 0; b < batch; ++b) {
+                PXD_DISPATCH_GS(launch_link, d_cclist2, nl, pnu);
                 if (heavy)
-                    k_cc_link_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nu, d_hrow, d_hoff, d_cclist2, d_row, d_dint,
-                                                                  d_nbr, d_part, d_root, d_cnt);
-                HIPC(hipGetLastError());
-                CHK(read_counters());
-                if (!hc.link_changed) break;
+                    k_cc_link_heavy<<<heavy_grid(), BS, 0, st>>>(&d_cnt->hchunks, d_hrow, d_hoff, d_cclist2, d_row,
+                                                                  d_dint, d_nbr, d_part, d_root, d_cnt,
+                                                                  &d_cnt->cc_done);
                 // shorten the trees the links deepened (finds stay exact without it)
-                for (int j = 0; j < 2; ++j) k_cc_jump<<<grid_for(nu, BS), BS, 0, st>>>(nu, d_cclist2, d_root);
-                // every changed round lowers a pointer, so this bound is never reached
-                if (round >= 100000)
-                    return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nu);
+                for (int j = 0; j < 2; ++j)
+                    k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist2, d_root, pnu, &d_cnt->cc_done);
+                k_cc_round<<<1, 64, 0, st>>>(d_cnt, ++r);
+                HIPC(hipGetLastError());
             }
+            uint32_t done = 0;
+            CHK(read_u32(&d_cnt->cc_done, &done));
+            if (done) break;
+            // every changed round lowers a pointer, so this bound is never reached
+            if (r >= 100000) return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nl);
         }
         return 0;                     // k_rid flattens every member while it reads the roots
     }
@@ -1516,8 +1520,10 @@ struct pxd_engine {
             CHK(walk());
             CHK(sweep());
             CHK(cut());
-            HIPC(hipMemcpyAsync(h_giter.data(), d_giter, G * 4, hipMemcpyDeviceToHost, st));
-            CHK(sync());
+            if (!giter_read) {
+                HIPC(hipMemcpyAsync(h_giter.data(), d_giter, G * 4, hipMemcpyDeviceToHost, st));
+                CHK(sync());
+            }
             for (uint32_t g = 0; g < G; ++g) {
                 Slot& S = slots[g];
                 if (!S.used || h_giter[g] == 0) continue;
@@ -1569,7 +1575,9 @@ struct pxd_engine {
     }
 };
 
-__global__ void k_mark_seen(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gseen) {
+__global__ void k_mark_seen(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gseen,
+                            const uint32_t* pn) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         if (c == 0 || cgraph[c] != cgraph[c - 1]) gseen[cgraph[c]] = 1u;    // first cluster of each graph
 }
@@ -1585,8 +1593,8 @@ __global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32
     }
 }
 
-void pxd_engine::k_mark_seen_launch(uint32_t n, const uint32_t* cgraph) {
-    k_mark_seen<<<grid_for(n, BS), BS, 0, st>>>(n, cgraph, d_gseen);
+void pxd_engine::k_mark_seen_launch(uint32_t n, const uint32_t* cgraph, const uint32_t* pn) {
+    k_mark_seen<<<grid_for(n, BS), BS, 0, st>>>(n, cgraph, d_gseen, pn);
 }
 
 // =================================================================== allocation-free CSR build

@@ -36,8 +36,19 @@ struct Counters {
     uint32_t link_changed;
     uint32_t admit_bad;
     uint32_t maxdeg;          // largest degree of an uploaded batch
-    uint32_t pad[6];
+    // counts that stay on the device within an iteration (their kernels read
+    // them; the host sizes grids from the iteration's upper bounds)
+    uint32_t nl, nu;          // CC members, members outside the giant trees
+    uint32_t hchunks;         // chunks of the current long-row map
+    uint32_t tiles, tiles256; // tiles of the large-segment sort plan (and x256)
+    uint32_t cc_done;         // the exact CC rounds have converged
+    uint32_t ccn;             // positions of the clusters that need CC (= CC members)
+    uint32_t ncr;             // clusters after the CC regroup
+    uint32_t na_new, nc_new;  // positions and clusters kept by the cut, read once per iteration
 };
+
+// n from the device when a count pointer is given, else the value passed
+__device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* p) { return p ? *p : n; }
 
 // ------------------------------------------------------------------ init
 
@@ -448,7 +459,8 @@ __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t
 constexpr uint32_t HEAVY_CHUNK = BS * 4;
 
 __global__ void k_heavy_count(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ dint,
-                              uint32_t heavy, uint32_t* __restrict__ hcnt) {
+                              uint32_t heavy, uint32_t* __restrict__ hcnt, const uint32_t* pn = nullptr) {
+    n = dev_count(n, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t d = dint[alist[i]];
         hcnt[i] = d > heavy ? (d + HEAVY_CHUNK - 1u) / HEAVY_CHUNK : 0u;
@@ -456,7 +468,8 @@ __global__ void k_heavy_count(uint32_t n, const uint32_t* __restrict__ alist, co
 }
 
 __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, const uint32_t* __restrict__ hoff,
-                               uint32_t* __restrict__ hrow) {
+                               uint32_t* __restrict__ hrow, const uint32_t* pn = nullptr) {
+    n = dev_count(n, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t c = hcnt[i], o = hoff[i];
         for (uint32_t k = 0; k < c; ++k) hrow[o + k] = i;
@@ -741,8 +754,9 @@ __global__ void __launch_bounds__(BS) k_contrib_heavy(const uint32_t* __restrict
 
 // out[i] = sum_{j<i} in[j] for i < n, out[n] = total (3-level reduce-then-scan).
 __global__ void __launch_bounds__(BS) k_scan_up(uint32_t n, const uint32_t* __restrict__ in,
-                                                uint32_t* __restrict__ bsum) {
+                                                uint32_t* __restrict__ bsum, const uint32_t* pn = nullptr) {
     __shared__ uint32_t lds[BS / WAVE];
+    n = dev_count(n, pn);             // blocks past n add nothing (their sums are 0)
     const uint32_t base = blockIdx.x * (BS * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
     uint32_t s = 0;
 #pragma unroll
@@ -754,8 +768,10 @@ __global__ void __launch_bounds__(BS) k_scan_up(uint32_t n, const uint32_t* __re
 
 __global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __restrict__ in,
                                                   uint32_t* __restrict__ out,
-                                                  const uint32_t* __restrict__ boff) {
+                                                  const uint32_t* __restrict__ boff, const uint32_t* pn = nullptr,
+                                                  uint32_t* total = nullptr) {
     __shared__ uint32_t lds[BS / WAVE];
+    n = dev_count(n, pn);
     const uint32_t base = blockIdx.x * (BS * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
@@ -768,7 +784,12 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __
         if (base + i < n) out[base + i] = run;
         run += v[i];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == BS - 1) out[n] = run;
+    // the block holding the last element (block 0 when n == 0) writes the total
+    const uint32_t last = n ? (n - 1) / (BS * SCAN_ITEMS) : 0u;
+    if (blockIdx.x == last && threadIdx.x == BS - 1) {
+        out[n] = run;
+        if (total) *total = run;
+    }
 }
 
 // Exclusive scan of up to SMALL_SCAN_N elements in one workgroup (one launch
@@ -776,8 +797,10 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t n, const uint32_t* __
 constexpr uint32_t SMALL_SCAN_ITEMS = 16;
 constexpr uint32_t SMALL_SCAN_N = 1024 * SMALL_SCAN_ITEMS;
 __global__ void __launch_bounds__(1024) k_scan_small(uint32_t n, const uint32_t* __restrict__ in,
-                                                     uint32_t* __restrict__ out) {
+                                                     uint32_t* __restrict__ out, const uint32_t* pn = nullptr,
+                                                     uint32_t* total = nullptr) {
     __shared__ uint32_t lds[1024 / WAVE];
+    n = dev_count(n, pn);
     const uint32_t base = threadIdx.x * SMALL_SCAN_ITEMS;
     uint32_t v[SMALL_SCAN_ITEMS];
     uint32_t s = 0;
@@ -793,7 +816,10 @@ __global__ void __launch_bounds__(1024) k_scan_small(uint32_t n, const uint32_t*
         if (base + i < n) out[base + i] = run;
         run += v[i];
     }
-    if (threadIdx.x == 0) out[n] = tot;
+    if (threadIdx.x == 0) {
+        out[n] = tot;
+        if (total) *total = tot;
+    }
 }
 
 // ------------------------------------------------------------------ segmented sort
@@ -818,12 +844,13 @@ __global__ void __launch_bounds__(BS) k_classify(uint32_t nseg, const uint32_t* 
     }
 }
 
-// one wave per segment (<= 64 items): rank by counting, stable on input order
-__global__ void __launch_bounds__(BS) k_sort_wave(uint32_t nlist, const uint32_t* __restrict__ list,
+// one wave per segment (<= 64 items): rank by counting, stable on input order;
+// waves stride over the list (its length read on the device)
+__global__ void __launch_bounds__(BS) k_sort_wave(const uint32_t* __restrict__ pnlist, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ segs,
                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
-    const uint32_t w = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE;
-    if (w >= nlist) return;
+  const uint32_t nlist = *pnlist;
+  for (uint32_t w = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE; w < nlist; w += gridDim.x * (BS / WAVE)) {
     const uint32_t lane = lane_id();
     const uint32_t c = list[w];
     const uint32_t s0 = segs[c], s = segs[c + 1] - s0;
@@ -839,6 +866,7 @@ __global__ void __launch_bounds__(BS) k_sort_wave(uint32_t nlist, const uint32_t
         key[s0 + rank] = k;
         val[s0 + rank] = v;
     }
+  }
 }
 
 // Stable rank of a tile's items by the 8-bit digit at `shift`.  Item i of
@@ -899,18 +927,20 @@ __device__ __forceinline__ void rank8(const uint64_t (&e)[TILE_ITEMS], uint32_t 
     __syncthreads();
 }
 
-// one block per segment of 65..TILE items, sorted in place
-__global__ void __launch_bounds__(BS) k_sort_block(uint32_t nlist, const uint32_t* __restrict__ list,
+// one block per segment of 65..TILE items, sorted in place; blocks stride
+// over the list (its length read on the device)
+__global__ void __launch_bounds__(BS) k_sort_block(const uint32_t* __restrict__ pnlist, const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ segs,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
     constexpr uint32_t NW = BS / WAVE;
     __shared__ uint64_t buf[TILE];
     __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t hs[256];
-    __shared__ uint32_t scratch[NW];
+    __shared__ uint32_t sred[NW];
     __shared__ uint32_t s_or, s_and;
-    if (blockIdx.x >= nlist) return;
-    const uint32_t c = list[blockIdx.x];
+    const uint32_t nlist = *pnlist;
+  for (uint32_t blk = blockIdx.x; blk < nlist; blk += gridDim.x) {
+    const uint32_t c = list[blk];
     const uint32_t s0 = segs[c], cnt = segs[c + 1] - s0;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
@@ -932,7 +962,7 @@ __global__ void __launch_bounds__(BS) k_sort_block(uint32_t nlist, const uint32_
     uint32_t pos[TILE_ITEMS];
     for (int pass = 0; pass < 4; ++pass) {
         if (((bits >> (8 * pass)) & 255u) == 0u) continue;     // a constant digit keeps the order
-        rank8(e, cnt, 8 * pass, wcnt, hs, scratch, pos);
+        rank8(e, cnt, 8 * pass, wcnt, hs, sred, pos);
 #pragma unroll
         for (int i = 0; i < TILE_ITEMS; ++i)
             if (w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l < cnt) buf[pos[i]] = e[i];
@@ -951,20 +981,55 @@ __global__ void __launch_bounds__(BS) k_sort_block(uint32_t nlist, const uint32_
             val[s0 + p] = (uint32_t)e[i];
         }
     }
+    __syncthreads();                  // s_or / s_and and buf are reused by the next segment
+  }
 }
 
-// large segments: tile descriptors (segment, tile index) are built on the host
+// large segments: tile descriptors (segment, tile index), built on the device
 struct LargeSeg {
     uint32_t start, len, ntiles, hbase;
 };
 
-__global__ void __launch_bounds__(BS) k_tile_hist(uint32_t ntiles, const uint32_t* __restrict__ tseg,
+// plan of the large segments, first pass: start, length and tile count of
+// each (the list's length read on the device)
+__global__ void k_large_plan(const uint32_t* __restrict__ pnlarge, const uint32_t* __restrict__ llarge,
+                             const uint32_t* __restrict__ segs, LargeSeg* __restrict__ ls,
+                             uint32_t* __restrict__ tcount) {
+    const uint32_t nl = *pnlarge;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t c = llarge[i];
+        const uint32_t st = segs[c], len = segs[c + 1] - st;
+        const uint32_t nt = (len + TILE - 1) / TILE;
+        ls[i] = LargeSeg{st, len, nt, 0u};
+        tcount[i] = nt;
+    }
+}
+
+// second pass, after the tile counts are scanned (toff): every tile's
+// segment and index, each segment's histogram base; tiles x 256 for the
+// histogram scans
+__global__ void k_large_expand(const uint32_t* __restrict__ pnlarge, const uint32_t* __restrict__ toff,
+                               LargeSeg* __restrict__ ls, uint32_t* __restrict__ tseg, uint32_t* __restrict__ tidx,
+                               Counters* cnt) {
+    const uint32_t nl = *pnlarge;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t o = toff[i], nt = ls[i].ntiles;
+        ls[i].hbase = o * 256u;
+        for (uint32_t t = 0; t < nt; ++t) {
+            tseg[o + t] = i;
+            tidx[o + t] = t;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt->tiles256 = toff[nl] * 256u;
+}
+
+__global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ pntiles, const uint32_t* __restrict__ tseg,
                                                   const uint32_t* __restrict__ tidx,
                                                   const LargeSeg* __restrict__ ls,
                                                   const uint32_t* __restrict__ key, int shift,
                                                   uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
-    if (blockIdx.x >= ntiles) return;
+    if (blockIdx.x >= *pntiles) return;   // uniform over the workgroup
     h[threadIdx.x] = 0u;
     __syncthreads();
     const LargeSeg L = ls[tseg[blockIdx.x]];
@@ -978,7 +1043,7 @@ __global__ void __launch_bounds__(BS) k_tile_hist(uint32_t ntiles, const uint32_
 
 // One LSD pass over one tile (rank8), then the scatter to the digit's global
 // offset: items are staged in LDS at their rank and written out in tile order.
-__global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint32_t* __restrict__ tseg,
+__global__ void __launch_bounds__(BS) k_tile_scatter(const uint32_t* __restrict__ pntiles, const uint32_t* __restrict__ tseg,
                                                      const uint32_t* __restrict__ tidx,
                                                      const LargeSeg* __restrict__ ls,
                                                      const uint32_t* __restrict__ hscan,
@@ -989,8 +1054,8 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint
     __shared__ uint64_t buf[TILE];
     __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t hs[256], goff[256];
-    __shared__ uint32_t scratch[NW];
-    if (blockIdx.x >= ntiles) return;
+    __shared__ uint32_t sred[NW];
+    if (blockIdx.x >= *pntiles) return;   // uniform over the workgroup
     const LargeSeg L = ls[tseg[blockIdx.x]];
     const uint32_t t = tidx[blockIdx.x];
     const uint32_t b = L.start + t * TILE;
@@ -1005,7 +1070,7 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(uint32_t ntiles, const uint
         e[i] = ok ? (((uint64_t)ksrc[b + p] << 32) | vsrc[b + p]) : 0ull;
     }
     uint32_t pos[TILE_ITEMS];
-    rank8(e, cnt, shift, wcnt, hs, scratch, pos);
+    rank8(e, cnt, shift, wcnt, hs, sred, pos);
 #pragma unroll
     for (int i = 0; i < TILE_ITEMS; ++i)
         if (w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l < cnt) buf[pos[i]] = e[i];
@@ -1223,7 +1288,7 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
 // clusters that need connected components this iteration.
 __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbest, double thr,
                          uint32_t* __restrict__ cflag, uint32_t* __restrict__ coff,
-                         float* __restrict__ csp, Counters* cnt) {
+                         float* __restrict__ csp, const uint32_t* __restrict__ cs, Counters* cnt) {
     for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += gridDim.x * blockDim.x) {
         const uint32_t c = c0 + threadIdx.x;
         uint32_t f = 0u;
@@ -1238,12 +1303,15 @@ __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbe
             if ((f & CF_CUT) || !(f & CF_CONN)) f |= CF_CC;
             cflag[c] = f;
         }
-        // counts: one atomic per wave
+        // counts: one atomic per wave; the CC clusters' sizes add up to the
+        // number of CC members of the cut
         const uint32_t ncut = (uint32_t)__popcll(__ballot(c < nc && (f & CF_CUT)));
         const uint32_t ncc = (uint32_t)__popcll(__ballot(c < nc && (f & CF_CC)));
+        const uint32_t ccn = group_sum<WAVE>((c < nc && (f & CF_CC)) ? cs[c + 1] - cs[c] : 0u);
         if (lane_id() == 0) {
             if (ncut) atomicAdd(&cnt->ncut, ncut);
             if (ncc) atomicAdd(&cnt->ncc, ncc);
+            if (ccn) atomicAdd(&cnt->ccn, ccn);
         }
     }
 }
@@ -1297,7 +1365,9 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
                                                   const uint32_t* __restrict__ dint,
                                                   const uint32_t* __restrict__ nbr,
                                                   const uint32_t* __restrict__ part,
-                                                  uint32_t* __restrict__ root, uint32_t r) {
+                                                  uint32_t* __restrict__ root, uint32_t r,
+                                                  const uint32_t* pn = nullptr) {
+    nl = dev_count(nl, pn);
     const XcdSpan sp = xcd_span(nl, BS, threadIdx.x);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
@@ -1323,8 +1393,10 @@ __global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32
                                                        const uint32_t* __restrict__ dint,
                                                        const uint32_t* __restrict__ nbr,
                                                        const uint32_t* __restrict__ part,
-                                                       uint32_t* __restrict__ root, uint32_t r, uint32_t N) {
+                                                       uint32_t* __restrict__ root, uint32_t r, uint32_t N,
+                                                       const uint32_t* pn = nullptr) {
     __shared__ uint32_t sp_[WIN], sr_[WIN];
+    nl = dev_count(nl, pn);
     const uint32_t nch = (nl + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
@@ -1403,7 +1475,8 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
 // f[i] = 1 for members outside their part's giant tree
 __global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
                               const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
-                              uint32_t* __restrict__ f) {
+                              uint32_t* __restrict__ f, const uint32_t* pn = nullptr) {
+    nl = dev_count(nl, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         f[i] = cc_find_stable(root, u) != giant[part[u]] ? 1u : 0u;
@@ -1425,7 +1498,10 @@ __global__ void __launch_bounds__(BS) k_cc_link(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ dint,
                                                 const uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ part,
-                                                uint32_t* __restrict__ root, uint32_t heavy, Counters* cnt) {
+                                                uint32_t* __restrict__ root, uint32_t heavy, Counters* cnt,
+                                                const uint32_t* pn = nullptr, const uint32_t* stop = nullptr) {
+    if (stop && *stop) return;        // the rounds have converged (uniform over the grid)
+    nl = dev_count(nl, pn);
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
     uint32_t changed = 0;
@@ -1461,7 +1537,9 @@ __global__ void __launch_bounds__(BS) k_cc_link_heavy(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ dint,
                                                       const uint32_t* __restrict__ nbr,
                                                       const uint32_t* __restrict__ part,
-                                                      uint32_t* __restrict__ root, Counters* cnt) {
+                                                      uint32_t* __restrict__ root, Counters* cnt,
+                                                      const uint32_t* stop = nullptr) {
+    if (stop && *stop) return;
     const uint32_t H = *nchunks;
     uint32_t changed = 0;
     for (uint32_t b = blockIdx.x; b < H; b += gridDim.x) {
@@ -1491,7 +1569,10 @@ __global__ void __launch_bounds__(BS) k_cc_link_heavy(const uint32_t* __restrict
 
 // pointer jumping, up to 8 steps per node and launch (own slot only)
 __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __restrict__ list,
-                                                uint32_t* __restrict__ root) {
+                                                uint32_t* __restrict__ root, const uint32_t* pn = nullptr,
+                                                const uint32_t* stop = nullptr) {
+    if (stop && *stop) return;
+    nl = dev_count(nl, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         const uint32_t r0 = ld_dev(&root[u]);
@@ -1548,7 +1629,9 @@ __global__ void k_build_clusters(uint32_t na, const uint32_t* __restrict__ ngrap
     }
 }
 
-__global__ void k_graph_first(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gfirst) {
+__global__ void k_graph_first(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gfirst,
+                              const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         if (c == 0 || cgraph[c] != cgraph[c - 1]) gfirst[cgraph[c]] = c;
 }
@@ -1556,7 +1639,9 @@ __global__ void k_graph_first(uint32_t nc, const uint32_t* __restrict__ cgraph, 
 // ------------------------------------------------------------------ cluster statistics
 
 __global__ void k_stats_reset(uint32_t nc, uint32_t* __restrict__ cmaxo, uint32_t* __restrict__ cmino,
-                              unsigned long long* __restrict__ csum, unsigned long long* __restrict__ cbest) {
+                              unsigned long long* __restrict__ csum, unsigned long long* __restrict__ cbest,
+                              const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         if (cmaxo) { cmaxo[c] = 0u; cmino[c] = 0xFFFFFFFFu; csum[c] = 0ull; }
         if (cbest) cbest[c] = ~0ull;
@@ -1568,7 +1653,8 @@ __global__ void k_stats_reset(uint32_t nc, uint32_t* __restrict__ cmaxo, uint32_
 // first keys
 __global__ void k_stats_ends(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ pnix,
                              const uint32_t* __restrict__ xkey, uint32_t* __restrict__ cmaxo,
-                             uint32_t* __restrict__ cmino) {
+                             uint32_t* __restrict__ cmino, const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         cmino[c] = xkey[pnix[cs[c]]];
         cmaxo[c] = xkey[pnix[cs[c + 1] - 1u]];
@@ -1581,7 +1667,9 @@ __device__ __forceinline__ int ceil_log2_u32(uint32_t k) {
 
 // fixed-point scale K per cluster: amax < 2^E, K = 62 - E - ceil(log2(count))
 __global__ void k_stats_scale(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cmaxo,
-                              const uint32_t* __restrict__ cmino, int32_t* __restrict__ cK) {
+                              const uint32_t* __restrict__ cmino, int32_t* __restrict__ cK,
+                              const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const float amax = fmaxf(fabsf(unord_f32(cmaxo[c])), fabsf(unord_f32(cmino[c])));
         int E = 0;
@@ -1651,7 +1739,8 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
 __global__ void k_stats_final(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cmaxo,
                               const uint32_t* __restrict__ cmino, const int32_t* __restrict__ cK,
                               const unsigned long long* __restrict__ csum, float rw_thr,
-                              uint32_t* __restrict__ ckeep, double* __restrict__ cavg) {
+                              uint32_t* __restrict__ ckeep, double* __restrict__ cavg, const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const uint32_t count = cs[c + 1] - cs[c];
         const float rw_sum = (float)ldexp(i64_to_f64((int64_t)csum[c]), -cK[c]);
@@ -1666,7 +1755,8 @@ __global__ void k_stats_final(uint32_t nc, const uint32_t* __restrict__ cs, cons
 __global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const uint32_t* __restrict__ gfirst,
                          const uint32_t* __restrict__ ckeep, const uint32_t* __restrict__ kscan,
                          const int32_t* __restrict__ gsmall, int32_t* __restrict__ gsmall_next,
-                         uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel) {
+                         uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel, const uint32_t* pn = nullptr) {
+    nc = dev_count(nc, pn);
     // a graph's clusters are contiguous: the per-graph survivor count and
     // smallest new label are reduced over the wave first (one atomic per graph
     // run in the wave instead of one per cluster on a single address)
@@ -1763,7 +1853,10 @@ __global__ void k_pos_compact(uint32_t na, const uint32_t* __restrict__ f, const
 
 __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __restrict__ ckeep,
                               const uint32_t* __restrict__ cscan, const uint32_t* __restrict__ pscan,
-                              ClusterCols in, ClusterCols out) {
+                              ClusterCols in, ClusterCols out, const uint32_t* pnc = nullptr,
+                              const uint32_t* pna_new = nullptr) {
+    nc = dev_count(nc, pnc);
+    na_new = dev_count(na_new, pna_new);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         if (!ckeep[c]) continue;
         const uint32_t d = cscan[c];
@@ -1784,7 +1877,8 @@ __global__ void k_alive_flags(uint32_t n, const uint32_t* __restrict__ alist, co
 }
 
 __global__ void k_compact_u32(uint32_t n, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,
-                              const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
+                              const uint32_t* __restrict__ in, uint32_t* __restrict__ out, const uint32_t* pn = nullptr) {
+    n = dev_count(n, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (f[i]) out[fs[i]] = in[i];
 }
@@ -1801,7 +1895,8 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
                                                 uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ root,
                                                 const double* __restrict__ x, double* __restrict__ y,
-                                                uint32_t heavy) {
+                                                uint32_t heavy, const uint32_t* pn = nullptr) {
+    nl = dev_count(nl, pn);
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
@@ -1851,9 +1946,10 @@ __global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t
                                                      uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ root,
                                                      const double* __restrict__ x, double* __restrict__ y,
-                                                     uint32_t heavy, uint32_t N) {
+                                                     uint32_t heavy, uint32_t N, const uint32_t* pn = nullptr) {
     static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of root quads, lane groups within a wave");
     __shared__ uint32_t sr[WIN];
+    nl = dev_count(nl, pn);
     constexpr uint32_t NPR = 64 / GS;
     constexpr uint32_t NW = WBS / 64;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
@@ -1939,7 +2035,9 @@ __global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t
 
 // 1 for the alive rows of the list above the threshold
 __global__ void k_heavy_flags(uint32_t n, const uint32_t* __restrict__ list, const uint8_t* __restrict__ alive,
-                              const uint32_t* __restrict__ dint, uint32_t heavy, uint32_t* __restrict__ f) {
+                              const uint32_t* __restrict__ dint, uint32_t heavy, uint32_t* __restrict__ f,
+                              const uint32_t* pn = nullptr) {
+    n = dev_count(n, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         f[i] = (alive[u] && dint[u] > heavy) ? 1u : 0u;
@@ -1991,8 +2089,10 @@ __global__ void __launch_bounds__(BS) k_disable_heavy(const uint32_t* __restrict
 // cvol[c] = sum of dInt over the cluster's positions (the k_stats_sum layout)
 __global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __restrict__ pnix,
                                                const uint32_t* __restrict__ pclu,
-                                               const uint32_t* __restrict__ dint, uint32_t* __restrict__ cvol) {
+                                               const uint32_t* __restrict__ dint, uint32_t* __restrict__ cvol,
+                                               const uint32_t* pn = nullptr) {
     __shared__ uint32_t lds[BS / WAVE];
+    na = dev_count(na, pn);
     constexpr uint32_t TR = BS * SUM_ITEMS;
     for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
         const uint32_t r1 = min(r0 + TR, na);
@@ -2039,8 +2139,26 @@ __global__ void k_gather(uint32_t n, const uint32_t* __restrict__ idx, const uin
         dst[i] = src[idx[i]];
 }
 
-__global__ void k_fill_u32(uint32_t n, uint32_t* __restrict__ p, uint32_t v) {
+__global__ void k_fill_u32(uint32_t n, uint32_t* __restrict__ p, uint32_t v, const uint32_t* pn = nullptr) {
+    n = dev_count(n, pn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// segment bounds {0, n} of a one-segment sort (n read on the device when given)
+__global__ void k_one_segment(uint32_t* __restrict__ seg, uint32_t n, const uint32_t* pn) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        seg[0] = 0u;
+        seg[1] = dev_count(n, pn);
+    }
+}
+
+// start of a batch of exact CC rounds: round r > 0 without a change ends them
+// (later launches of the batch see cc_done and return at once)
+__global__ void k_cc_round(Counters* cnt, uint32_t r) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !cnt->cc_done) {
+        if (r > 0 && cnt->link_changed == 0u) cnt->cc_done = 1u;
+        else cnt->link_changed = 0u;
+    }
 }
 
 // count of still-valid arcs over all nodes (getNumCutEdges)

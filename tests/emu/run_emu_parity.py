@@ -29,6 +29,9 @@ GRAPHS = {
     "path100": lambda: px.from_edges(100, np.arange(99), np.arange(1, 100)),
     "sbm300": lambda: px.sbm(300, 3, 0.06, 0.003, 100),
     "torus12": lambda: px.torus(12, 3, 1),
+    # 2500 positions: the CC regroup sorts one segment of more than a tile
+    # (the large-segment plan built on the device, several tiles)
+    "sbm2500": lambda: px.sbm(2500, 2, 0.01, 0.0004, 7),
 }
 
 

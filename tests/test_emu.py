@@ -1,7 +1,8 @@
 """The engine's own kernel and host sources, compiled for the CPU by the
 emulator (tests/emu: one thread per lane, barriers for workgroup and wave
 exchanges) with AddressSanitizer, decompose small graphs end to end and
-equal the oracle, and the device CSR build of an edge list (loops,
+equal the oracle (2500 nodes: segments above a sort tile, sorted by the
+tile plan the device builds from its own counts), and the device CSR build of an edge list (loops,
 duplicates, a row longer than a workgroup's tile) equals the host build.
 Every device array is a host allocation there, so an
 access out of bounds -- such as a wave wholly past the end of the position
@@ -31,7 +32,7 @@ def test_emulated_engine_with_address_sanitizer_matches_oracle():
     import build_emu
     lib = build_emu.build(asan=True)
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0")
-    cases = ["path100", "sbm300", "ingest_loops_dups", "ingest_hub3000"]
+    cases = ["path100", "sbm300", "sbm2500", "ingest_loops_dups", "ingest_hub3000"]
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_emu_parity.py"), lib, *cases],
                        capture_output=True, text=True, timeout=800, env=env)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-6000:]
