@@ -174,9 +174,12 @@ struct pxd_engine {
     LargeSeg* d_ls = nullptr;
     uint32_t *d_tseg = nullptr, *d_tidx = nullptr, *d_hist = nullptr, *d_hscan = nullptr;
     uint32_t tiles_cap = 0, ls_cap = 0;
+    // single-pass scans: status words, ticket counter, the epoch of the last launch
+    unsigned long long* d_lb_scan = nullptr;
+    uint32_t* d_lb_ticket = nullptr;
+    uint32_t lb_scan_cap = 0, lb_epoch = 0;
     uint32_t *d_tail_ed = nullptr, *d_tail_vol = nullptr, *d_hasst = nullptr, *d_carry_ed = nullptr,
              *d_carry_vol = nullptr;
-    std::vector<uint32_t*> lvl_sum, lvl_scan;
     uint32_t* d_oneseg = nullptr;
     uint32_t* d_gclu = nullptr;
     uint32_t* d_giant = nullptr;
@@ -256,8 +259,6 @@ struct pxd_engine {
         for (size_t i = 0; i < allocs.size(); ++i) pool.emplace_back(alloc_bytes[i], allocs[i]);
         allocs.clear();
         alloc_bytes.clear();
-        lvl_sum.clear();
-        lvl_scan.clear();
         d_hcnt = d_hoff = d_hrow = nullptr;   // heavy-row buffers are allocated lazily
         d_hpart = nullptr;
         heavy_cap = 0;
@@ -400,25 +401,17 @@ struct pxd_engine {
         CHK(alloc(&d_tidx, tiles_cap));
         CHK(alloc(&d_hist, (size_t)tiles_cap * 256 + 1));
         CHK(alloc(&d_hscan, (size_t)tiles_cap * 256 + 2));
+        // the largest scans: N + 1 elements, or the tile histograms
+        lb_scan_cap = cdiv(std::max<uint64_t>((uint64_t)n1 + 1, (uint64_t)tiles_cap * 256 + 1), LB_SCAN_TILE) + 1;
+        CHK(alloc(&d_lb_scan, lb_scan_cap));
+        CHK(alloc(&d_lb_ticket, 1));
         const uint32_t nbk = cdiv(n1, BS * SCAN_ITEMS) + 1;
         CHK(alloc(&d_tail_ed, nbk));
         CHK(alloc(&d_tail_vol, nbk));
         CHK(alloc(&d_hasst, nbk));
         CHK(alloc(&d_carry_ed, nbk));
         CHK(alloc(&d_carry_vol, nbk));
-        {
-            uint64_t len = std::max<uint64_t>(n1 + 1, (uint64_t)tiles_cap * 256 + 2);
-            while (true) {
-                uint32_t nb = cdiv(len, BS * SCAN_ITEMS);
-                uint32_t *s = nullptr, *c = nullptr;
-                CHK(alloc(&s, nb + 1));
-                CHK(alloc(&c, nb + 2));
-                lvl_sum.push_back(s);
-                lvl_scan.push_back(c);
-                if (nb <= 1) break;
-                len = nb;
-            }
-        }
+        CHK(lb_clear());
         CHK(alloc(&d_oneseg, 2));
         CHK(alloc(&d_gclu, G));
         CHK(alloc(&d_giant, 2 * n1 + 2));
@@ -636,38 +629,39 @@ struct pxd_engine {
     }
 
     // ---------------------------------------------------------------- scan
-    int scan_level(const uint32_t* in, uint32_t* out, uint32_t n, size_t lvl, const uint32_t* pn, uint32_t* total) {
-        const uint32_t nb = cdiv(n, BS * SCAN_ITEMS);
-        if (nb <= 1) {
-            k_scan_down<<<1, BS, 0, st>>>(n, in, out, nullptr, pn, total);
-            HIPC(hipGetLastError());
-            return 0;
-        }
-        if (lvl >= lvl_sum.size()) return set_err(PXD_ERR_STATE, "scan too deep");
-        // the block sums cover the bound n (blocks past the device count add 0)
-        k_scan_up<<<nb, BS, 0, st>>>(n, in, lvl_sum[lvl], pn);
-        HIPC(hipGetLastError());
-        CHK(scan_level(lvl_sum[lvl], lvl_scan[lvl], nb, lvl + 1, nullptr, nullptr));
-        k_scan_down<<<nb, BS, 0, st>>>(n, in, out, lvl_scan[lvl], pn, total);
-        HIPC(hipGetLastError());
+    // Status words and tickets start cleared (buffers may come back from the
+    // pool holding anything); epochs then keep launches apart.
+    int lb_clear() {
+        HIPC(hipMemsetAsync(d_lb_scan, 0, (size_t)lb_scan_cap * 8, st));
+        HIPC(hipMemsetAsync(d_lb_ticket, 0, sizeof(uint32_t), st));
+        lb_epoch = 0;
+        return 0;
+    }
+    int next_epoch(uint32_t* e) {
+        if (lb_epoch == LB_EPOCH_MAX) CHK(lb_clear());
+        *e = ++lb_epoch;
         return 0;
     }
     // out[0..n) exclusive prefix sums, out[n] = total.  With pn the length is
-    // *pn, read on the device, and n is a bound on it that sizes the grids;
+    // *pn, read on the device, and n is a bound on it that sizes the grid;
     // total (when given) receives the sum too, at an address the host knows.
     int scan(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_t* pn = nullptr, uint32_t* total = nullptr) {
         if (n == 0) {
             k_fill_u32<<<1, 64, 0, st>>>(1, out, 0u);
             if (total) k_fill_u32<<<1, 64, 0, st>>>(1, total, 0u);
-            HIPC(hipGetLastError());
-            return 0;
-        }
-        if (n > BS * SCAN_ITEMS && n <= SMALL_SCAN_N) {
+        } else if (n <= (uint32_t)(BS * SCAN_ITEMS)) {
+            k_scan_down<<<1, BS, 0, st>>>(n, in, out, nullptr, pn, total);
+        } else if (n <= SMALL_SCAN_N) {
             k_scan_small<<<1, 1024, 0, st>>>(n, in, out, pn, total);
-            HIPC(hipGetLastError());
-            return 0;
+        } else {
+            const uint32_t nb = cdiv(n, LB_SCAN_TILE);
+            if (nb > lb_scan_cap) return set_err(PXD_ERR_STATE, "scan of %u elements exceeds the status capacity", n);
+            uint32_t ep;
+            CHK(next_epoch(&ep));
+            k_scan_lb<<<nb, BS, 0, st>>>(n, in, out, d_lb_scan, d_lb_ticket, ep, pn, total);
         }
-        return scan_level(in, out, n, 0, pn, total);
+        HIPC(hipGetLastError());
+        return 0;
     }
 
     // ---------------------------------------------------------------- segmented sort
@@ -699,7 +693,7 @@ struct pxd_engine {
             k_large_plan<<<grid_for(nlarge, BS), BS, 0, st>>>(&d_cnt->nlarge, d_llarge, segs, d_ls, d_ltiles);
             HIPC(hipGetLastError());
             CHK(scan(d_ltiles, d_ltoff, nlarge, &d_cnt->nlarge, &d_cnt->tiles));
-            k_large_expand<<<grid_for(nlarge, BS), BS, 0, st>>>(&d_cnt->nlarge, d_ltoff, d_ls, d_tseg, d_tidx, d_cnt);
+            k_large_expand<<<grid_for(tiles, BS), BS, 0, st>>>(&d_cnt->nlarge, d_ltoff, d_ls, d_tseg, d_tidx, d_cnt);
             HIPC(hipGetLastError());
             int src = 0;
             for (int pass = 0; pass < 4; ++pass) {   // four passes: the result is back in buffer 0

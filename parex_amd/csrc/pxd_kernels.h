@@ -822,6 +822,126 @@ __global__ void __launch_bounds__(1024) k_scan_small(uint32_t n, const uint32_t*
     }
 }
 
+// ------------------------------------------------------------------ decoupled look-back
+
+// Status words of the single-pass kernels below (Merrill & Garland, "Single-
+// pass Parallel Prefix Scan with Decoupled Look-back", 2016): bits 0-31 a
+// count, bits 32-33 the state (LB_AGG = the tile's own total, LB_INC = the
+// inclusive prefix up to and including the tile), bits 34-63 the launch's
+// epoch.  A word of another epoch reads as not written yet, so the status
+// arrays are never cleared between launches (only once, when allocated).
+// Tiles are handed out in ticket order (an atomic counter), so a tile only
+// waits for tiles whose workgroups have started, and those publish without
+// waiting for later ones: every wait ends.  The workgroup that draws the
+// grid's last ticket sets the counter back to 0 for the next launch.
+constexpr uint32_t LB_AGG = 1u, LB_INC = 2u;
+constexpr uint32_t LB_EPOCH_MAX = (1u << 30) - 1u;
+
+__device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, uint32_t state, uint32_t v) {
+    return ((unsigned long long)epoch << 34) | ((unsigned long long)state << 32) | v;
+}
+__device__ __forceinline__ void lb_publish(unsigned long long* p, unsigned long long w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long lb_read(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// state of a word in this epoch (0 = not written yet)
+__device__ __forceinline__ uint32_t lb_state(unsigned long long w, uint32_t epoch) {
+    return (uint32_t)(w >> 34) == epoch ? (uint32_t)(w >> 32) & 3u : 0u;
+}
+// called by one thread of the workgroup
+__device__ __forceinline__ uint32_t lb_ticket(uint32_t* ticket) {
+    const uint32_t t = atomicAdd(ticket, 1u);
+    if (t == gridDim.x - 1u) atomicExch(ticket, 0u);   // every workgroup has drawn its ticket
+    return t;
+}
+
+// Single-pass exclusive scan: out[i] = sum_{j<i} in[j], out[n] = total (also
+// to *total when given); n = *pn when pn is given, the grid then covers the
+// bound n.  One tile of LB_SCAN_TILE elements per workgroup.  A status hop
+// costs a device-scope load (about a microsecond across the XCDs), and the
+// inclusive prefixes advance by one look-back window per hop, so tiles are
+// large (8192 elements) and the whole workgroup reads a window of BS earlier
+// tiles at a time.
+constexpr uint32_t LB_SCAN_ITEMS = 32;
+constexpr uint32_t LB_SCAN_TILE = BS * LB_SCAN_ITEMS;
+static_assert(LB_SCAN_ITEMS % 32u == 0u || 32u % LB_SCAN_ITEMS == 0u, "padded LDS index of a thread's run");
+__global__ void __launch_bounds__(BS) k_scan_lb(uint32_t n, const uint32_t* __restrict__ in,
+                                                uint32_t* __restrict__ out, unsigned long long* status,
+                                                uint32_t* ticket, uint32_t epoch, const uint32_t* pn = nullptr,
+                                                uint32_t* total = nullptr) {
+    constexpr uint32_t NW = BS / WAVE;
+    // the tile in LDS, element k at k + k / 32: coalesced loads and stores
+    // in element order, each thread's LB_SCAN_ITEMS consecutive elements
+    // read without bank conflicts
+    __shared__ uint32_t sm[LB_SCAN_TILE + LB_SCAN_TILE / 32];
+    __shared__ uint32_t lds[NW];
+    __shared__ uint32_t s_tile, s_first;
+    if (threadIdx.x == 0) s_tile = lb_ticket(ticket);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    n = dev_count(n, pn);
+    const uint32_t ntiles = n ? (n - 1u) / LB_SCAN_TILE + 1u : 1u;   // tile 0 writes out[0] when n == 0
+    if (tile >= ntiles) return;
+    const uint32_t base = tile * LB_SCAN_TILE;
+#pragma unroll
+    for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
+        const uint32_t k = i * BS + threadIdx.x;
+        sm[k + (k >> 5)] = base + k < n ? in[base + k] : 0u;
+    }
+    __syncthreads();
+    const uint32_t k0 = threadIdx.x * LB_SCAN_ITEMS + threadIdx.x * LB_SCAN_ITEMS / 32u;
+    uint32_t v[LB_SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
+        v[i] = sm[k0 + i + i / 32u];
+        s += v[i];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(s, lds, tot);
+    if (threadIdx.x == 0) lb_publish(&status[tile], lb_word(epoch, tile == 0 ? LB_INC : LB_AGG, tot));
+    uint32_t prefix = 0;
+    if (tile != 0) {
+        // windows of BS earlier tiles; the nearest inclusive one ends the walk
+        for (int64_t top = (int64_t)tile - 1;; top -= BS) {
+            const int64_t j = top - (int64_t)threadIdx.x;
+            unsigned long long w = 0ull;
+            uint32_t stt = LB_INC;    // tiles before 0 read as an inclusive prefix of 0
+            if (j >= 0) {
+                while ((stt = lb_state(w = lb_read(&status[j]), epoch)) == 0u) __builtin_amdgcn_s_sleep(1);
+            }
+            if (threadIdx.x == 0) s_first = BS;
+            __syncthreads();
+            if (stt == LB_INC) atomicMin(&s_first, threadIdx.x);
+            __syncthreads();
+            const uint32_t first = s_first;
+            uint32_t part;
+            (void)block_excl_scan<BS>(threadIdx.x <= first && j >= 0 ? (uint32_t)w : 0u, lds, part);
+            prefix += part;
+            if (first < BS) break;
+        }
+        if (threadIdx.x == 0) lb_publish(&status[tile], lb_word(epoch, LB_INC, prefix + tot));
+    }
+    uint32_t run = prefix + ex;
+#pragma unroll
+    for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
+        sm[k0 + i + i / 32u] = run;
+        run += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
+        const uint32_t k = i * BS + threadIdx.x;
+        if (base + k < n) out[base + k] = sm[k + (k >> 5)];
+    }
+    if (tile == ntiles - 1u && threadIdx.x == 0) {
+        out[n] = prefix + tot;
+        if (total) *total = prefix + tot;
+    }
+}
+
 // ------------------------------------------------------------------ segmented sort
 
 // Sort classes: size <= 1 nothing, <= 64 one wave, <= TILE one block in LDS,
@@ -985,13 +1105,16 @@ __global__ void __launch_bounds__(BS) k_sort_block(const uint32_t* __restrict__ 
   }
 }
 
-// large segments: tile descriptors (segment, tile index), built on the device
+// Large segments (more than TILE items): LSD radix sort over the four 8-bit
+// digits, per digit a histogram of every tile, an exclusive scan of the
+// histograms (segment by segment, digit-major) and the scatter.  The tile
+// plan is built on the device.
 struct LargeSeg {
     uint32_t start, len, ntiles, hbase;
 };
 
-// plan of the large segments, first pass: start, length and tile count of
-// each (the list's length read on the device)
+// plan, first pass: start, length and tile count of every large segment (the
+// list's length read on the device)
 __global__ void k_large_plan(const uint32_t* __restrict__ pnlarge, const uint32_t* __restrict__ llarge,
                              const uint32_t* __restrict__ segs, LargeSeg* __restrict__ ls,
                              uint32_t* __restrict__ tcount) {
@@ -1005,22 +1128,26 @@ __global__ void k_large_plan(const uint32_t* __restrict__ pnlarge, const uint32_
     }
 }
 
-// second pass, after the tile counts are scanned (toff): every tile's
-// segment and index, each segment's histogram base; tiles x 256 for the
-// histogram scans
+// second pass, after the tile counts are scanned (toff, toff[nl] = tiles):
+// every segment's histogram base, and the segment and index of every tile
+// (the segment by binary search over toff)
 __global__ void k_large_expand(const uint32_t* __restrict__ pnlarge, const uint32_t* __restrict__ toff,
                                LargeSeg* __restrict__ ls, uint32_t* __restrict__ tseg, uint32_t* __restrict__ tidx,
                                Counters* cnt) {
     const uint32_t nl = *pnlarge;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
-        const uint32_t o = toff[i], nt = ls[i].ntiles;
-        ls[i].hbase = o * 256u;
-        for (uint32_t t = 0; t < nt; ++t) {
-            tseg[o + t] = i;
-            tidx[o + t] = t;
+    const uint32_t T = toff[nl];
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt->tiles256 = T * 256u;   // length of the histogram scans
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < T; j += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nl;     // last i with toff[i] <= j
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (toff[mid] <= j) lo = mid;
+            else hi = mid;
         }
+        tseg[j] = lo;
+        tidx[j] = j - toff[lo];
+        if (j == toff[lo]) ls[lo].hbase = j * 256u;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt->tiles256 = toff[nl] * 256u;
 }
 
 __global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ pntiles, const uint32_t* __restrict__ tseg,

@@ -118,6 +118,8 @@ float erfcinvf(float y);
 
 template <class T> inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicOr(T* p, T v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+template <class T> inline T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
 template <class T> inline T atomicAnd(T* p, T v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline T atomicMin(T* p, T v) {
     T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
