@@ -247,6 +247,7 @@ struct pxd_engine {
         if (tev0) (void)hipEventDestroy(tev0);
         if (tev1) (void)hipEventDestroy(tev1);
         if (st) (void)hipStreamDestroy(st);
+        if (h_pin) (void)hipHostFree(h_pin);
     }
 
     // The input's buffers go to the pool.  Nothing queued on the engine's
@@ -306,31 +307,32 @@ struct pxd_engine {
         return 0;
     }
 
+    // pinned host words for the reads of device counts (a direct copy, no staging)
+    uint32_t* h_pin = nullptr;
+    static constexpr size_t PIN_BYTES = 256;
+    int pinned() {
+        if (!h_pin) HIPC(hipHostMalloc((void**)&h_pin, PIN_BYTES, hipHostMallocDefault));
+        return 0;
+    }
     int read_counters() {
-        HIPC(hipMemcpyAsync(&hc, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        static_assert(sizeof(Counters) + 16 <= PIN_BYTES, "pinned read buffer");
+        CHK(pinned());
+        HIPC(hipMemcpyAsync(h_pin, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
         CHK(sync());
+        std::memcpy(&hc, h_pin, sizeof(Counters));
         if (hc.err & ERR_FIX_RANGE)
             return set_err(PXD_ERR_NUMERIC, "walk value outside the exact fixed-point range (|x/dInt| >= 2^31)");
         return 0;
     }
 
     int read_u32(const uint32_t* d, uint32_t* h) {
-        HIPC(hipMemcpyAsync(h, d, 4, hipMemcpyDeviceToHost, st));
-        return sync();
-    }
-    // two device words with one sync
-    int read_u32x2(const uint32_t* d0, uint32_t* h0, const uint32_t* d1, uint32_t* h1) {
-        HIPC(hipMemcpyAsync(h0, d0, 4, hipMemcpyDeviceToHost, st));
-        HIPC(hipMemcpyAsync(h1, d1, 4, hipMemcpyDeviceToHost, st));
-        return sync();
-    }
-
-    int zero_counter_fields() {
-        // counters other than the sticky error word
-        HIPC(hipMemsetAsync(((char*)d_cnt) + 4, 0, sizeof(Counters) - 4, st));
+        CHK(pinned());
+        uint32_t* slot = h_pin + (PIN_BYTES - 16) / 4;
+        HIPC(hipMemcpyAsync(slot, d, 4, hipMemcpyDeviceToHost, st));
+        CHK(sync());
+        *h = *slot;
         return 0;
     }
-
     // device arrays for N nodes, A arcs and G graphs (capacities when serving)
     int allocate() {
         const size_t n1 = (size_t)N + 1;
@@ -671,6 +673,11 @@ struct pxd_engine {
     // and every kernel reads its list length from the counters.
     int seg_sort(const uint32_t* segs, uint32_t nseg, uint32_t n) {
         if (nseg == 0 || n < 2) return 0;
+        if (n <= SORT_SMALL_USE) {    // one workgroup sorts every segment
+            k_sort_small<<<1, SS_NT, 0, st>>>(nseg, segs, d_pkey[0], d_pnix[0]);
+            HIPC(hipGetLastError());
+            return 0;
+        }
         HIPC(hipMemsetAsync(&d_cnt->nwave, 0, 3 * sizeof(uint32_t), st));
         k_classify<<<grid_for(nseg, BS), BS, 0, st>>>(nseg, segs, d_lwave, d_lblock, d_llarge, d_cnt);
         HIPC(hipGetLastError());
@@ -948,7 +955,7 @@ struct pxd_engine {
                                                           d_nbr, d_xkey, d_ediff);
             HIPC(hipGetLastError());
         }
-        k_stats_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, nullptr, nullptr, nullptr, d_cbest);
+        k_sweep_reset<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cbest, d_cnt);
         const uint32_t nb = cdiv(na, BS * SCAN_ITEMS);
         k_sweep_a<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, d_ediff, d_ped, d_pvol, d_tail_ed,
                                      d_tail_vol, d_hasst);
@@ -958,7 +965,6 @@ struct pxd_engine {
         k_sweep_c<<<nb, BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T.cs, T.cvol, d_ped, d_pvol, d_carry_ed, d_carry_vol,
                                      d_cbest, P.record_node_data ? d_nd : nullptr);
         HIPC(hipGetLastError());
-        CHK(zero_counter_fields());
         k_decide<<<grid_for(nc, BS), BS, 0, st>>>(nc, d_cbest, (double)P.sc_threshold, T.cflag, d_coff, d_csp, T.cs,
                                                    d_cnt);
         HIPC(hipGetLastError());
@@ -1017,29 +1023,26 @@ struct pxd_engine {
         }
         // statistics of the (possibly new) clusters
         const uint32_t gc = grid_for(ncb, BS);
-        k_graph_first<<<gc, BS, 0, st>>>(ncb, T->cgraph, d_gfirst, pnc);
-        k_stats_reset<<<gc, BS, 0, st>>>(ncb, d_cmaxo, d_cmino, d_csum, nullptr, pnc);
         // positions of every cluster are in ascending walk-key order (sweep sort,
         // then the stable regroup): min and max are the first and last keys
-        k_stats_ends<<<gc, BS, 0, st>>>(ncb, T->cs, d_pnix[0], d_xkey, d_cmaxo, d_cmino, pnc);
-        k_stats_scale<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, pnc);
+        k_stats_prep<<<gc, BS, 0, st>>>(ncb, T->cs, T->cgraph, d_pnix[0], d_xkey, d_gfirst, d_cmaxo, d_cmino, d_cK,
+                                        d_csum, pnc);
         k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
         k_stats_final<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold, d_ckeep, d_cavg,
                                          pnc);
         HIPC(hipGetLastError());
         CHK(scan(d_ckeep, d_kscan, ncb, pnc, &d_cnt->nc_new));
         k_labels<<<gc, BS, 0, st>>>(ncb, T->cgraph, d_gfirst, d_ckeep, d_kscan, d_gsmall, d_gsmall_next, d_gsurv,
-                                    T->clabel, pnc);
-        k_mark_seen_launch(ncb, T->cgraph, pnc);
+                                    T->clabel, d_gseen, pnc);
         k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter,
                                                        d_gstart, iter1);
         // the active node list holds the same nodes as the positions: apply in index order
-        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_nclu);
+        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_ckeep, d_nclu, d_tA);
         k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
                                                            d_dint, d_label, d_x[cur], d_y[cur], d_alive);
         HIPC(hipGetLastError());
         // drop deactivated clusters from the position list, the cluster table and the node list
-        k_pos_keepflags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pclu[0], d_ckeep, d_tA);
+        // (keep flags per position from k_node_cluster)
         CHK(scan(d_tA, d_tB, na, nullptr, &d_cnt->na_new));
         k_pos_compact<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_pclu[0], d_kscan, d_pnix[1],
                                                        d_pclu[1]);
@@ -1084,16 +1087,14 @@ struct pxd_engine {
             }
             HIPC(hipGetLastError());
         }
-        if (nc) {
+        if (nc) {                     // k_clu_compact cleared the volumes
             ClusterTable& T3 = ct[ctc];
-            k_fill_u32<<<grid_for(nc, BS), BS, 0, st>>>(nc, T3.cvol, 0u);
             k_volume<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
             HIPC(hipGetLastError());
         }
         return 0;
     }
 
-    void k_mark_seen_launch(uint32_t n, const uint32_t* cgraph, const uint32_t* pn = nullptr);
 
     // Components of every CC part (clusters flagged CF_CC in table T):
     // sampling rounds over the first arcs of each node, each followed by one
@@ -1124,7 +1125,7 @@ struct pxd_engine {
             k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
         }
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
-        k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC);
+        k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC, d_cnt);
         HIPC(hipGetLastError());
         CHK(scan(d_tC, d_tA, nl));
         const uint32_t* pnu = d_tA + nl;      // members outside the giant trees (<= nl)
@@ -1139,8 +1140,6 @@ struct pxd_engine {
             k_heavy_expand<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_hcnt, d_hoff, d_hrow, pnu);
             HIPC(hipGetLastError());
         }
-        HIPC(hipMemsetAsync(&d_cnt->cc_done, 0, 4, st));
-        HIPC(hipMemsetAsync(&d_cnt->link_changed, 0, 4, st));
         uint32_t r = 0;
         for (uint32_t batch = 3;; batch = std::min(2 * batch, 24u)) {
             for (uint32_t b = 0; b < batch; ++b) {
@@ -1150,8 +1149,7 @@ struct pxd_engine {
                                                                   d_dint, d_nbr, d_part, d_root, d_cnt,
                                                                   &d_cnt->cc_done);
                 // shorten the trees the links deepened (finds stay exact without it)
-                for (int j = 0; j < 2; ++j)
-                    k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist2, d_root, pnu, &d_cnt->cc_done);
+                k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist2, d_root, pnu, &d_cnt->cc_done);
                 k_cc_round<<<1, 64, 0, st>>>(d_cnt, ++r);
                 HIPC(hipGetLastError());
             }
@@ -1569,13 +1567,6 @@ struct pxd_engine {
     }
 };
 
-__global__ void k_mark_seen(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gseen,
-                            const uint32_t* pn) {
-    nc = dev_count(nc, pn);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
-        if (c == 0 || cgraph[c] != cgraph[c - 1]) gseen[cgraph[c]] = 1u;    // first cluster of each graph
-}
-
 __global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32_t* __restrict__ dint,
                           double* __restrict__ x, double* __restrict__ y, uint32_t* __restrict__ xkey) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
@@ -1587,9 +1578,6 @@ __global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32
     }
 }
 
-void pxd_engine::k_mark_seen_launch(uint32_t n, const uint32_t* cgraph, const uint32_t* pn) {
-    k_mark_seen<<<grid_for(n, BS), BS, 0, st>>>(n, cgraph, d_gseen, pn);
-}
 
 // =================================================================== allocation-free CSR build
 

@@ -1105,6 +1105,150 @@ __global__ void __launch_bounds__(BS) k_sort_block(const uint32_t* __restrict__ 
   }
 }
 
+// Stable rank of NT * ITEMS items by an 8-bit digit per item: the item
+// layout and the ballot ranking of rank8 (item i of lane l of wave w is
+// position w*WAVE*ITEMS + i*WAVE + l, valid below cnt), with the digits given
+// by the caller.  Block-wide; ends with a barrier.
+template <int NT, int ITEMS>
+__device__ __forceinline__ void rank_digits(const uint32_t (&dg)[ITEMS], uint32_t cnt, uint32_t (*wcnt)[256],
+                                            uint32_t* hs, uint32_t* scratch, uint32_t (&pos)[ITEMS]) {
+    constexpr uint32_t NW = NT / WAVE;
+    static_assert(NT >= 256, "one thread per digit");
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    for (uint32_t q = threadIdx.x; q < NW * 256u; q += NT) (&wcnt[0][0])[q] = 0u;
+    __syncthreads();
+    const uint64_t lt = (1ull << l) - 1ull;
+    uint32_t rk[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t d = dg[i];
+        uint64_t peer = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool one = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(one);
+            peer &= one ? bb : ~bb;
+        }
+        const bool lead = ok && (peer & lt) == 0ull;
+        const uint32_t leader = ok ? (uint32_t)__builtin_ctzll(peer) : l;
+        const uint32_t c0 = lead ? wcnt[w][d] : 0u;
+        const uint32_t c = __shfl(c0, (int)leader);
+        if (lead) wcnt[w][d] = c0 + (uint32_t)__popcll(peer);
+        rk[i] = c + (uint32_t)__popcll(peer & lt);
+    }
+    __syncthreads();
+    uint32_t run = 0;
+    if (threadIdx.x < 256u) {
+        const uint32_t d = threadIdx.x;   // wcnt[w][d] becomes the count of digit d in waves < w
+#pragma unroll
+        for (uint32_t ww = 0; ww < NW; ++ww) {
+            const uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = run;
+            run += c;
+        }
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<NT>(run, scratch, tot);
+    if (threadIdx.x < 256u) hs[threadIdx.x] = ex;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) pos[i] = hs[dg[i] & 255u] + wcnt[w][dg[i] & 255u] + rk[i];
+    __syncthreads();
+}
+
+// Segmented stable sort of up to SORT_SMALL_N items [segs[0], segs[nseg]) in
+// one workgroup (one launch instead of the size classes and the tile passes):
+// every item gets a label that grows with its segment (the number of segment
+// starts at or before it), then LSD passes over the key digits that vary and
+// over the label digits sort by (label, key); stability keeps key ties in
+// input order.
+constexpr uint32_t SS_NT = 512, SS_ITEMS = 8;
+constexpr uint32_t SORT_SMALL_N = SS_NT * SS_ITEMS;
+constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;   // sorts of up to this many items take k_sort_small
+__global__ void __launch_bounds__(SS_NT) k_sort_small(uint32_t nseg, const uint32_t* __restrict__ segs,
+                                                      uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+    constexpr uint32_t NW = SS_NT / WAVE;
+    __shared__ uint64_t buf[SORT_SMALL_N];
+    __shared__ uint32_t lab[SORT_SMALL_N];
+    __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t hs[256];
+    __shared__ uint32_t sred[NW];
+    __shared__ uint32_t s_or, s_and;
+    const uint32_t base = segs[0];
+    const uint32_t cnt = min(segs[nseg] - base, SORT_SMALL_N);   // the host keeps it within the bound
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    for (uint32_t q = threadIdx.x; q < SORT_SMALL_N; q += SS_NT) lab[q] = 0u;
+    if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < nseg; c += SS_NT) {
+        const uint32_t st = segs[c] - base;
+        if (st < cnt) atomicAdd(&lab[st], 1u);
+    }
+    __syncthreads();
+    {   // inclusive scan of the start counts, SS_ITEMS consecutive entries a thread
+        const uint32_t q0 = threadIdx.x * SS_ITEMS;
+        uint32_t v[SS_ITEMS], sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < SS_ITEMS; ++i) { v[i] = lab[q0 + i]; sum += v[i]; }
+        uint32_t tot;
+        uint32_t run = block_excl_scan<SS_NT>(sum, sred, tot);
+#pragma unroll
+        for (uint32_t i = 0; i < SS_ITEMS; ++i) { run += v[i]; lab[q0 + i] = run; }
+    }
+    __syncthreads();
+    uint64_t e[SS_ITEMS];
+    uint32_t lb[SS_ITEMS];
+    uint32_t lor = 0u, land = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < (int)SS_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t k = ok ? key[base + p] : 0u;
+        e[i] = ((uint64_t)k << 32) | (ok ? val[base + p] : 0u);
+        lb[i] = ok ? lab[p] : 0u;
+        if (ok) { lor |= k; land &= k; }
+    }
+    atomicOr(&s_or, lor);
+    atomicAnd(&s_and, land);
+    __syncthreads();
+    const uint32_t kbits = s_or ^ s_and;                      // key bits that vary
+    const uint32_t lbits = nseg > 1u ? 32u - (uint32_t)__clz((int)nseg) : 0u;   // labels are <= nseg
+    for (int pass = 0; pass < 8; ++pass) {
+        const bool on_label = pass >= 4;
+        const int sh = 8 * (pass & 3);
+        if (!on_label && ((kbits >> sh) & 255u) == 0u) continue;   // a constant digit keeps the order
+        if (on_label && (uint32_t)sh >= lbits) break;
+        uint32_t dg[SS_ITEMS], pos[SS_ITEMS];
+#pragma unroll
+        for (int i = 0; i < (int)SS_ITEMS; ++i)
+            dg[i] = on_label ? (lb[i] >> sh) & 255u : (uint32_t)(e[i] >> (32 + sh)) & 255u;
+        rank_digits<SS_NT, SS_ITEMS>(dg, cnt, wcnt, hs, sred, pos);
+#pragma unroll
+        for (int i = 0; i < (int)SS_ITEMS; ++i)
+            if (w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l < cnt) {
+                buf[pos[i]] = e[i];
+                lab[pos[i]] = lb[i];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < (int)SS_ITEMS; ++i) {
+            const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
+            if (p < cnt) { e[i] = buf[p]; lb[i] = lab[p]; }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < (int)SS_ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
+        if (p < cnt) {
+            key[base + p] = (uint32_t)(e[i] >> 32);
+            val[base + p] = (uint32_t)e[i];
+        }
+    }
+}
+
 // Large segments (more than TILE items): LSD radix sort over the four 8-bit
 // digits, per digit a histogram of every tile, an exclusive scan of the
 // histograms (segment by segment, digit-major) and the scatter.  The tile
@@ -1602,8 +1746,11 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
 // f[i] = 1 for members outside their part's giant tree
 __global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
                               const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
-                              uint32_t* __restrict__ f, const uint32_t* pn = nullptr) {
-    nl = dev_count(nl, pn);
+                              uint32_t* __restrict__ f, Counters* cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the exact rounds that follow start unconverged
+        cnt->cc_done = 0u;
+        cnt->link_changed = 0u;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
         const uint32_t u = list[i];
         f[i] = cc_find_stable(root, u) != giant[part[u]] ? 1u : 0u;
@@ -1756,35 +1903,15 @@ __global__ void k_build_clusters(uint32_t na, const uint32_t* __restrict__ ngrap
     }
 }
 
-__global__ void k_graph_first(uint32_t nc, const uint32_t* __restrict__ cgraph, uint32_t* __restrict__ gfirst,
-                              const uint32_t* pn = nullptr) {
-    nc = dev_count(nc, pn);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
-        if (c == 0 || cgraph[c] != cgraph[c - 1]) gfirst[cgraph[c]] = c;
-}
-
 // ------------------------------------------------------------------ cluster statistics
 
-__global__ void k_stats_reset(uint32_t nc, uint32_t* __restrict__ cmaxo, uint32_t* __restrict__ cmino,
-                              unsigned long long* __restrict__ csum, unsigned long long* __restrict__ cbest,
-                              const uint32_t* pn = nullptr) {
-    nc = dev_count(nc, pn);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        if (cmaxo) { cmaxo[c] = 0u; cmino[c] = 0xFFFFFFFFu; csum[c] = 0ull; }
-        if (cbest) cbest[c] = ~0ull;
-    }
-}
-
-// max / min of (float)x over each cluster as ordered bits (-0 < +0): the
-// positions of a cluster are sorted by that key, so they are its last and
-// first keys
-__global__ void k_stats_ends(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ pnix,
-                             const uint32_t* __restrict__ xkey, uint32_t* __restrict__ cmaxo,
-                             uint32_t* __restrict__ cmino, const uint32_t* pn = nullptr) {
-    nc = dev_count(nc, pn);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        cmino[c] = xkey[pnix[cs[c]]];
-        cmaxo[c] = xkey[pnix[cs[c + 1] - 1u]];
+// before the sweep: best cut of every cluster unset, and the counters the
+// sweep and the cut fill cleared (all but the sticky error word)
+__global__ void k_sweep_reset(uint32_t nc, unsigned long long* __restrict__ cbest, Counters* cnt) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) cbest[c] = ~0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
+        for (uint32_t i = 1; i < sizeof(Counters) / 4; ++i) w[i] = 0u;
     }
 }
 
@@ -1792,16 +1919,28 @@ __device__ __forceinline__ int ceil_log2_u32(uint32_t k) {
     return k <= 1u ? 0 : 32 - __clz((int)(k - 1u));
 }
 
-// fixed-point scale K per cluster: amax < 2^E, K = 62 - E - ceil(log2(count))
-__global__ void k_stats_scale(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cmaxo,
-                              const uint32_t* __restrict__ cmino, int32_t* __restrict__ cK,
-                              const uint32_t* pn = nullptr) {
+// Per cluster before its sums: first cluster of every graph (gfirst); max /
+// min of (float)x as ordered bits (-0 < +0) -- the positions of a cluster are
+// sorted by that key, so they are its last and first keys; the fixed-point
+// scale K (amax < 2^E, K = 62 - E - ceil(log2(count))); the sum cleared.
+__global__ void k_stats_prep(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cgraph,
+                             const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ xkey,
+                             uint32_t* __restrict__ gfirst, uint32_t* __restrict__ cmaxo,
+                             uint32_t* __restrict__ cmino, int32_t* __restrict__ cK,
+                             unsigned long long* __restrict__ csum, const uint32_t* pn = nullptr) {
     nc = dev_count(nc, pn);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const float amax = fmaxf(fabsf(unord_f32(cmaxo[c])), fabsf(unord_f32(cmino[c])));
+        const uint32_t g = cgraph[c];
+        if (c == 0 || g != cgraph[c - 1]) gfirst[g] = c;
+        const uint32_t b = cs[c], e = cs[c + 1];
+        const uint32_t lo = xkey[pnix[b]], hi = xkey[pnix[e - 1u]];
+        cmino[c] = lo;
+        cmaxo[c] = hi;
+        const float amax = fmaxf(fabsf(unord_f32(hi)), fabsf(unord_f32(lo)));
         int E = 0;
         if (amax != 0.0f) frexpf(amax, &E);
-        cK[c] = 62 - E - ceil_log2_u32(cs[c + 1] - cs[c]);
+        cK[c] = 62 - E - ceil_log2_u32(e - b);
+        csum[c] = 0ull;
     }
 }
 
@@ -1882,7 +2021,8 @@ __global__ void k_stats_final(uint32_t nc, const uint32_t* __restrict__ cs, cons
 __global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const uint32_t* __restrict__ gfirst,
                          const uint32_t* __restrict__ ckeep, const uint32_t* __restrict__ kscan,
                          const int32_t* __restrict__ gsmall, int32_t* __restrict__ gsmall_next,
-                         uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel, const uint32_t* pn = nullptr) {
+                         uint32_t* __restrict__ gsurv, int32_t* __restrict__ clabel, uint32_t* __restrict__ gseen,
+                         const uint32_t* pn = nullptr) {
     nc = dev_count(nc, pn);
     // a graph's clusters are contiguous: the per-graph survivor count and
     // smallest new label are reduced over the wave first (one atomic per graph
@@ -1895,6 +2035,7 @@ __global__ void k_labels(uint32_t nc, const uint32_t* __restrict__ cgraph, const
         int32_t lab = 0x7FFFFFFF;
         if (ok) {
             const uint32_t f = gfirst[g];
+            if (c == f) gseen[g] = 1u;    // the graph has clusters this iteration
             if (ckeep[c]) {
                 clabel[c] = (int32_t)(kscan[c] - kscan[f]);
                 surv = 1u;
@@ -1933,11 +2074,16 @@ __global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive,
     }
 }
 
-// cluster of every active node: nclu[pnix[k]] = pclu[k]
+// per position: the cluster of its node (nclu[pnix[k]] = pclu[k]) and
+// whether that cluster stays active
 __global__ void k_node_cluster(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                               uint32_t* __restrict__ nclu) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
-        nclu[pnix[k]] = pclu[k];
+                               const uint32_t* __restrict__ ckeep, uint32_t* __restrict__ nclu,
+                               uint32_t* __restrict__ f) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pclu[k];
+        nclu[pnix[k]] = c;
+        f[k] = ckeep[c];
+    }
 }
 
 // per active node, in index order: write labels, recentre survivors
@@ -1964,12 +2110,6 @@ __global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, co
 
 // ------------------------------------------------------------------ compaction
 
-__global__ void k_pos_keepflags(uint32_t na, const uint32_t* __restrict__ pclu, const uint32_t* __restrict__ ckeep,
-                                uint32_t* __restrict__ f) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
-        f[k] = ckeep[pclu[k]];
-}
-
 __global__ void k_pos_compact(uint32_t na, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,
                               const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
                               const uint32_t* __restrict__ cscan, uint32_t* __restrict__ pnix_o,
@@ -1992,7 +2132,7 @@ __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __re
         out.cmin[d] = in.cmin[c];
         out.cflag[d] = in.cflag[c] & CF_CONN;
         out.clabel[d] = in.clabel[c];
-        out.cvol[d] = in.cvol[c];
+        out.cvol[d] = 0u;             // k_volume adds the new volumes
         if (d + 1 == cscan[nc]) out.cs[d + 1] = na_new;
     }
 }

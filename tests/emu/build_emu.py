@@ -2,8 +2,8 @@
 
 The engine source is compiled unchanged except that every kernel launch
 `k<<<grid, block, shmem, stream>>>(args)` is rewritten into a call of the
-emulator's launcher, and that the scans' size thresholds are lowered so
-that small graphs take the multi-tile single-pass scan.  Output: tests/emu/_build/libpxd_emu.so (and
+emulator's launcher, and that the scans' and sorts' size thresholds are
+lowered so that small graphs also take the multi-tile paths.  Output: tests/emu/_build/libpxd_emu.so (and
 libpxd_emu_asan.so with AddressSanitizer when asan=True).
 """
 import os
@@ -71,12 +71,14 @@ def build(asan=False):
     # generated source, which its quoted includes find first
     for h in ("pxd_ingest.h",):
         open(os.path.join(OUT, h), "w").write(transform(open(os.path.join(CSRC, h)).read()))
-    # the kernel header with small look-back scan tiles (1024 elements) and no
-    # one-workgroup scans above 1024 elements, so that the small graphs of the
-    # parity runs already take the single-pass scan over several tiles
+    # the kernel header with small look-back scan tiles (1024 elements), no
+    # one-workgroup scans above 1024 elements and one-workgroup sorts only up
+    # to 1024 items, so that the small graphs of the parity runs already take
+    # the single-pass scan over several tiles and the size-class sort
     kh = open(os.path.join(CSRC, "pxd_kernels.h")).read()
     for old, new in (("constexpr uint32_t SMALL_SCAN_N = 1024 * SMALL_SCAN_ITEMS;", "constexpr uint32_t SMALL_SCAN_N = 0;"),
-                     ("constexpr uint32_t LB_SCAN_ITEMS = 32;", "constexpr uint32_t LB_SCAN_ITEMS = 4;")):
+                     ("constexpr uint32_t LB_SCAN_ITEMS = 32;", "constexpr uint32_t LB_SCAN_ITEMS = 4;"),
+                     ("constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;", "constexpr uint32_t SORT_SMALL_USE = 1024;")):
         assert kh.count(old) == 1, old
         kh = kh.replace(old, new)
     open(os.path.join(OUT, "pxd_kernels.h"), "w").write(kh)
