@@ -395,17 +395,19 @@ def batched_leg(pinned, warmup, steps):
         log(f"step {k}: {step_ms[-1]:.1f} ms = upload+init {init_ms[-1]:.1f} + decomposition {dec_ms[-1]:.1f} "
             f"({dm.lastRunMs():.1f} ms device), iterations {it}")
     ids, ks = dm.getFinalPartition()
+    ks = np.atleast_1d(ks)            # one count per graph (a scalar for one graph)
     its = dm.graphIterations()
     # the walk kernel's roofline: one more decomposition with every walk launch timed
     dm.initialize_batch(pinned)
     dm.enableWalkTiming(True)
     dm.expanderDecomposition()
     walk = dm.walkStats()
+    walk_log = dm.walkLaunchLog()
     dm.enableWalkTiming(False)
     tuning = dm.walkTuning()
     dm.close()
     return {"step_ms": float(np.mean(step_ms)), "init_ms": float(np.mean(init_ms)), "dec_ms": float(np.mean(dec_ms)),
-            "ids": ids, "ks": ks, "its": its, "walk": walk, "tuning": tuning}
+            "ids": ids, "ks": ks, "its": its, "walk": walk, "walk_log": walk_log, "tuning": tuning}
 
 
 def plain_leg(pinned, batch):
@@ -485,6 +487,14 @@ def main():
                 # node-range chunks
                 "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
                 "tunings": tunings}
+    lms, lby = b["walk_log"]
+    if len(lms):
+        big = lby >= 0.5 * lby.max()        # launches over at least half the largest one's bytes
+        roofline["launches_at_half_max_bytes"] = {
+            "count": int(big.sum()), "bytes_share": round(float(lby[big].sum() / lby.sum()), 4),
+            "frac": round(float(lby[big].sum() / (lms[big].sum() / 1e3) / 1e9 / HBM_PEAK_GBS), 4)}
+        roofline["launch_us"] = [round(float(v) * 1e3, 1) for v in lms]
+        roofline["launch_mb"] = [round(float(v) / 1e6, 1) for v in lby]
 
     plain, same = None, None
     if not a.no_plain:

@@ -153,6 +153,17 @@ int pxd_last_run_ms(pxd_engine* e, double* ms);
  * bytes (4+8 per valid arc + 32 per active node) summed over those launches. */
 int pxd_enable_walk_timing(pxd_engine* e, int enable);
 int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* alg_bytes);
+/* The same per timed launch, in launch order: ms[i] and alg_bytes[i] for
+ * i < *count (count alone when both arrays are null). */
+int pxd_walk_launch_log(pxd_engine* e, double* ms, double* alg_bytes, uint32_t capacity, uint32_t* count);
+
+/* One-workgroup runs (no counterpart in the reference, whose decomposition
+ * loop is driven from the host, cuda/device.h:301-350): pxd_expander_decomposition
+ * of a single graph of at most 4096 nodes and 2^19 arcs, started right after
+ * pxd_initialize or pxd_reset, runs every iteration inside one workgroup in
+ * one launch, with the same results as the per-phase kernels.  enable = 1
+ * (default) allows it, 0 keeps the per-phase kernels. */
+int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
 
 /* Kernel choice for the walk and the cut contribution, the two per-arc
  * passes of an iteration (no counterpart in the reference, whose kernels have

@@ -131,7 +131,9 @@ def pxd() -> C.CDLL:
         lib.pxd_last_run_ms.argtypes = [P, C.POINTER(C.c_double)]
         lib.pxd_enable_walk_timing.argtypes = [P, C.c_int]
         lib.pxd_walk_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.pxd_walk_launch_log.argtypes = [P, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
         lib.pxd_set_walk_variant.argtypes = [P, C.c_int]
+        lib.pxd_set_small_graph_kernel.argtypes = [P, C.c_int]
         lib.pxd_walk_tuning.argtypes = [P, _i32p, C.c_uint32, C.POINTER(C.c_uint64)]
         lib.pxd_serve_create.argtypes = [C.POINTER(_Params), C.c_uint32, C.c_uint64, C.c_uint32,
                                          C.POINTER(C.c_void_p)]
@@ -160,6 +162,7 @@ PXD_SYMBOLS = [
     "pxd_download_labels", "pxd_download_graph", "pxd_download_active_edge_map", "pxd_download_partition",
     "pxd_read_sweep_cuts", "pxd_get_num_cut_edges", "pxd_download_active_nodes", "pxd_get_counts",
     "pxd_set_random_walk_values_f64", "pxd_last_run_ms", "pxd_enable_walk_timing", "pxd_walk_stats",
+    "pxd_walk_launch_log", "pxd_set_small_graph_kernel",
     "pxd_set_walk_variant", "pxd_walk_tuning", "pxd_initialize_edges", "pxd_serve_enable_edges",
     "pxd_serve_admit_edges",
     "pxd_serve_create", "pxd_serve_admit", "pxd_serve_step", "pxd_serve_next", "pxd_serve_wait", "pxd_serve_take",
@@ -585,6 +588,20 @@ class DeviceManager:
         a, b, c = C.c_uint64(), C.c_double(), C.c_double()
         self._check(self._lib.pxd_walk_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return int(a.value), float(b.value), float(c.value)
+
+    def walkLaunchLog(self):
+        """(ms, algorithmic bytes) of every timed walk launch, in launch order."""
+        n = C.c_uint32()
+        self._check(self._lib.pxd_walk_launch_log(self._h, None, None, 0, C.byref(n)))
+        ms, by = np.zeros(n.value), np.zeros(n.value)
+        self._check(self._lib.pxd_walk_launch_log(self._h, ms.ctypes.data_as(C.c_void_p),
+                                                   by.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return ms, by
+
+    def setSmallGraphKernel(self, enable: bool):
+        """Allow (default) or forbid the one-workgroup run of a small single
+        graph (at most 4096 nodes, 2^19 arcs) in expanderDecomposition."""
+        self._check(self._lib.pxd_set_small_graph_kernel(self._h, int(enable)))
 
     def setWalkVariant(self, variant: int):
         """Pin the kernels of the walk and the cut contribution (0..5: 2..64

@@ -44,23 +44,32 @@ __device__ __forceinline__ void fix_add128(Fix128& a, uint64_t blo, uint64_t bhi
     a.lo = lo;
 }
 
-// Adds trunc(y * 2^72) (truncation toward zero) to the accumulator, without
-// branches.  |y| * 2^72 = M * 2^sh with the 53-bit significand M: for sh < 0
-// the magnitude is M >> -sh (truncated; zero and subnormal y shift out
-// entirely), the sign is applied to that 64-bit value, and for sh > 0 the
-// signed value is shifted left across the two words, which is exact.  Terms
-// with |y| >= 2^31 (sh > 50, also inf/nan) set `ovf` and leave the sum
-// meaningless.
-__device__ __forceinline__ void fix_add_term(Fix128& a, double y, uint32_t& ovf) {
+// Walk terms.  The term of y is trunc(y * 2^72) (truncation toward zero).
+// With |y| * 2^72 = M * 2^sh (53-bit significand M) it is M >> -sh for sh < 0
+// (zero and subnormal y shift out entirely) and M << sh for sh >= 0, signed.
+// It is computed once by the kernel that writes y and stored in one 64-bit
+// word: q = (t << 6) | l with the term equal to t * 2^l, t signed, |t| < 2^53,
+// 0 <= l <= 50.  A zero word adds nothing.  |y| >= 2^31 (sh > 50, also inf
+// and nan) sets `ovf`: the engine then reports an error instead of a sum.
+// Readers (fix_add_packed) only shift and add.
+__device__ __forceinline__ uint64_t fix_pack(double y, uint32_t& ovf) {
     const uint64_t b = (uint64_t)__double_as_longlong(y);
     const int E = (int)((b >> 52) & 0x7FFu);
     const uint64_t M = (b & ((1ull << 52) - 1)) | (1ull << 52);
     const int sh = E - (1075 - FIX_SHIFT);
     ovf |= (uint32_t)(sh > 50);
-    const uint32_t l = (uint32_t)min(max(sh, 0), 63);
+    const uint32_t l = (uint32_t)min(max(sh, 0), 50);
     const uint32_t r = (uint32_t)min(max(-sh, 0), 63);
     const int64_t m = (int64_t)b >> 63;                            // 0 or -1
-    const int64_t t = (int64_t)(((M >> r) ^ (uint64_t)m) - (uint64_t)m);   // signed, |t| < 2^53
+    const int64_t t = (int64_t)(((M >> r) ^ (uint64_t)m) - (uint64_t)m);
+    return ((uint64_t)t << 6) | l;
+}
+
+// Adds a packed term (fix_pack) to the accumulator: t << l across the two
+// words, exact.
+__device__ __forceinline__ void fix_add_packed(Fix128& a, uint64_t q) {
+    const uint32_t l = (uint32_t)q & 63u;
+    const int64_t t = (int64_t)q >> 6;
     const uint64_t qlo = (uint64_t)t << l;
     const uint64_t qhi = (uint64_t)((t >> 1) >> (63 - l));         // t >> (64 - l), sign-filled
     fix_add128(a, qlo, qhi);
@@ -144,6 +153,10 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// The value stays in its VGPRs as computed: the optimizer cannot fold the
+// load that produced it into a later select (no instruction is emitted).
+__device__ __forceinline__ void opaque_u64(uint64_t& v) { __asm__ volatile("" : "+v"(v)); }
 
 // XCD-aware split of a grid-stride loop over n items.  Workgroups are
 // dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8, each XCD

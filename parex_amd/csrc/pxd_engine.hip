@@ -22,6 +22,7 @@
 #include "parex_amd/pxd.h"
 #include "pxd_kernels.h"
 #include "pxd_ingest.h"
+#include "pxd_small.h"
 
 using namespace pxd;
 
@@ -143,7 +144,7 @@ struct pxd_engine {
     uint32_t* d_dint = nullptr;
     int32_t* d_label = nullptr;
     double* d_x[2] = {nullptr, nullptr};
-    double* d_y[2] = {nullptr, nullptr};
+    uint64_t* d_y[2] = {nullptr, nullptr};   // packed walk terms of x/dInt (fix_pack)
     uint32_t* d_xkey = nullptr;
     uint2* d_ediff = nullptr;   // per node: cut contribution and dInt of the current sweep
     uint32_t *d_ped = nullptr, *d_pvol = nullptr;   // cut contribution and dInt in position order (sweep)
@@ -177,6 +178,7 @@ struct pxd_engine {
     // single-pass scans: status words, ticket counter, the epoch of the last launch
     unsigned long long* d_lb_scan = nullptr;
     uint32_t* d_lb_ticket = nullptr;
+    uint32_t* d_sout = nullptr;        // final counts of the one-workgroup run
     uint32_t lb_scan_cap = 0, lb_epoch = 0;
     uint32_t *d_tail_ed = nullptr, *d_tail_vol = nullptr, *d_hasst = nullptr, *d_carry_ed = nullptr,
              *d_carry_vol = nullptr;
@@ -210,6 +212,7 @@ struct pxd_engine {
     bool walk_timing = false;
     uint64_t walk_launches = 0;
     double walk_ms = 0.0, walk_bytes = 0.0;
+    std::vector<double> walk_log_ms, walk_log_bytes;   // per timed launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // kernel choice for the two arc passes of an iteration (0 = walk, 1 = cut
@@ -232,6 +235,7 @@ struct pxd_engine {
     bool rmap_valid = false;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
+    bool small_kernel = true;          // one-workgroup runs for small single graphs
     int tuned[2][33];
     uint64_t tuned_for[3] = {~0ull, ~0ull, ~0ull};   // (graphs, nodes, arcs) the tables were tuned on
     uint64_t walk_tunings = 0;
@@ -419,6 +423,7 @@ struct pxd_engine {
         CHK(alloc(&d_giant, 2 * n1 + 2));
         CHK(alloc(&d_cclist2, N));
         CHK(alloc(&d_u64, 1));
+        CHK(alloc(&d_sout, 8));
         CHK(alloc(&d_cnt, 1));
         return 0;
     }
@@ -601,7 +606,7 @@ struct pxd_engine {
         if (N) {
             k_init_nodes<<<grid_for(N, BS), BS, 0, st>>>(N, G, d_gnode, d_row, P.seed, d_deg, d_dint, d_x[0], d_y[0],
                                                           d_xkey, d_label, d_alive, d_stamp, d_pnix[0], d_alist[0],
-                                                          d_pclu[0], d_gclu, d_ngraph);
+                                                          d_pclu[0], d_gclu, d_ngraph, &d_cnt->err);
             HIPC(hipGetLastError());
         }
         CHK(sync());
@@ -858,18 +863,21 @@ struct pxd_engine {
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(cdiv(nl, CHUNK), 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
         if (lanes == 1)
             k_disable_win<1, 4, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
-                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N,
+                                                                    &d_cnt->err);
         else if (lanes == 2)
             k_disable_win<2, 4, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
-                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N,
+                                                                    &d_cnt->err);
         else
             k_disable_win<4, 4, 32768, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
-                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N);
+                                                                    d_root, d_x[cur], d_y[cur], heavy_thr, N,
+                                                                    &d_cnt->err);
     }
     template <int GSZ>
     void launch_disable(uint32_t nl) {
         k_disable<GSZ><<<grid_xcd(nl, BS / GSZ), BS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr, d_root,
-                                                                d_x[cur], d_y[cur], heavy_thr);
+                                                                d_x[cur], d_y[cur], heavy_thr, &d_cnt->err);
     }
 
     // chunk map of the active rows above heavy_thr (valid until the next cut)
@@ -927,6 +935,8 @@ struct pxd_engine {
                 walk_launches++;
                 walk_ms += ms;
                 walk_bytes += 12.0 * (double)arcs + 32.0 * (double)nalist;
+                walk_log_ms.push_back(ms);
+                walk_log_bytes.push_back(12.0 * (double)arcs + 32.0 * (double)nalist);
             }
             cur ^= 1;                 // the walk wrote x' and y' = x'/dInt into the other buffers
         } else {
@@ -1039,7 +1049,7 @@ struct pxd_engine {
         // the active node list holds the same nodes as the positions: apply in index order
         k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_ckeep, d_nclu, d_tA);
         k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
-                                                           d_dint, d_label, d_x[cur], d_y[cur], d_alive);
+                                                           d_dint, d_label, d_x[cur], d_y[cur], d_alive, &d_cnt->err);
         HIPC(hipGetLastError());
         // drop deactivated clusters from the position list, the cluster table and the node list
         // (keep flags per position from k_node_cluster)
@@ -1083,7 +1093,7 @@ struct pxd_engine {
                 CHK(scan(d_hcnt, d_hoff, nl));
                 k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_hcnt, d_hoff, d_cclist, d_hrow);
                 k_disable_heavy<<<heavy_grid(), BS, 0, st>>>(d_hoff + nl, d_hrow, d_row, d_dint, d_nbr, d_root,
-                                                              d_x[cur], d_y[cur]);
+                                                              d_x[cur], d_y[cur], &d_cnt->err);
             }
             HIPC(hipGetLastError());
         }
@@ -1478,7 +1488,7 @@ struct pxd_engine {
             k_admit_nodes<<<grid_for(Q.n, BS), BS, 0, st>>>(Q.base, Q.n, Q.abase, Q.slot, nc, d_stage + Q.stage, P.seed,
                                                             d_row, d_deg, d_dint, d_x[cur], d_y[cur], d_xkey, d_label,
                                                             d_alive, d_stamp, d_ngraph, d_pnix[0] + na,
-                                                            d_pclu[0] + na, d_alist[acur] + nalist);
+                                                            d_pclu[0] + na, d_alist[acur] + nalist, &d_cnt->err);
             if (Q.arcs) k_rebase<<<grid_for(Q.arcs, BS), BS, 0, st>>>(Q.arcs, d_nbr + Q.abase, Q.base);
             k_admit_cluster<<<1, 64, 0, st>>>(nc, na, Q.n, Q.slot, Q.base, Q.arcs, iter1, ct[ctc].cols(), d_gsmall,
                                               d_gsmall_next, d_gsurv, d_giter, d_gseen, d_gstart);
@@ -1536,17 +1546,99 @@ struct pxd_engine {
         return PXD_OK;
     }
 
+    // One graph of at most SMALL_MAX_N nodes, straight after initialize or
+    // reset, with the per-arc kernels left to the engine's choice and no
+    // per-launch walk timing: the whole run is one workgroup's launch.
+    bool small_run_applies() const {
+        return small_kernel && !serving && G == 1 && N > 0 && N <= SMALL_MAX_N && A <= SMALL_MAX_ARCS &&
+               walk_force < 0 && !walk_timing && iter1 == 0 && cur == 0 && acur == 0 && ctc == 0 &&
+               na == N && nalist == N && nc == 1 && !sweep_done;
+    }
+    int run_small(uint32_t* loops) {
+        SmallArgs sa{};
+        sa.n = N;
+        sa.maxit = P.max_iter;
+        sa.na = na;
+        sa.nc = nc;
+        sa.nalist = nalist;
+        sa.sc_thr = (double)P.sc_threshold;
+        sa.rw_thr = P.rw_threshold;
+        sa.row = d_row;
+        sa.dint = d_dint;
+        sa.nbr = d_nbr;
+        sa.xkey = d_xkey;
+        sa.stamp = d_stamp;
+        sa.root = d_root;
+        sa.x[0] = d_x[0];
+        sa.x[1] = d_x[1];
+        sa.y[0] = d_y[0];
+        sa.y[1] = d_y[1];
+        sa.label = d_label;
+        sa.alive = d_alive;
+        for (int b = 0; b < 2; ++b) {
+            sa.pnix[b] = d_pnix[b];
+            sa.pclu[b] = d_pclu[b];
+            sa.alist[b] = d_alist[b];
+            sa.ct[b] = ct[b].cols();
+        }
+        sa.pkey = d_pkey[0];
+        sa.cclist = d_cclist;
+        sa.nclu = d_nclu;
+        sa.flags = d_tB;
+        sa.edv = d_ediff;
+        sa.coff = d_coff;
+        sa.ckeep = d_ckeep;
+        sa.kscan = d_kscan;
+        sa.cmaxo = d_cmaxo;
+        sa.cmino = d_cmino;
+        sa.csp = d_csp;
+        sa.cavg = d_cavg;
+        sa.cK = d_cK;
+        sa.gsmall = d_gsmall;
+        sa.giter = d_giter;
+        sa.nd = P.record_node_data ? d_nd : nullptr;
+        sa.cnt = d_cnt;
+        sa.out = d_sout;
+        k_decompose_small<<<1, SM_NT, 0, st>>>(sa);
+        HIPC(hipGetLastError());
+        CHK(pinned());
+        static_assert(sizeof(Counters) + 64 <= PIN_BYTES, "pinned words of the run's counts");
+        uint32_t* o = h_pin + (PIN_BYTES - 64) / 4;
+        HIPC(hipMemcpyAsync(o, d_sout, 8 * 4, hipMemcpyDeviceToHost, st));
+        CHK(read_counters());         // syncs; reports a walk value out of range
+        na = o[0];
+        nc = o[1];
+        nalist = o[2];
+        cur = (int)o[3];
+        iter1 = o[4];
+        *loops = o[5];
+        if (o[6]) {                   // the current positions in buffer 0, as after a cut
+            std::swap(d_pnix[0], d_pnix[1]);
+            std::swap(d_pclu[0], d_pclu[1]);
+        }
+        ctc = (int)(o[7] >> 1);
+        acur = (int)(o[7] & 1u);
+        heavy_valid = false;
+        rmap_valid = false;
+        sweep_done = false;
+        return 0;
+    }
+
     int run(int32_t* iterations) {
         if (!inited) return set_err(PXD_ERR_ARG, "engine not initialized");
         if (serving) return set_err(PXD_ERR_ARG, "a serving engine advances with pxd_serve_step");
         HIPC(hipEventRecord(ev0, st));
         const uint32_t maxit = P.max_iter;
         uint32_t i = 0;
-        while (i++ < maxit && nc > 0 && na > 0) {
-            CHK(walk());
-            CHK(sweep());
-            if (na == 0) break;
-            CHK(cut());
+        if (small_run_applies()) {
+            CHK(run_small(&i));
+        } else {
+            while (i++ < maxit && nc > 0 && na > 0) {
+                CHK(walk());
+                CHK(sweep());
+                if (na == 0) break;
+                CHK(cut());
+            }
         }
         HIPC(hipEventRecord(ev1, st));
         HIPC(hipEventSynchronize(ev1));
@@ -1568,12 +1660,12 @@ struct pxd_engine {
 };
 
 __global__ void k_set_all(uint32_t n, const double* __restrict__ v, const uint32_t* __restrict__ dint,
-                          double* __restrict__ x, double* __restrict__ y, uint32_t* __restrict__ xkey) {
+                          double* __restrict__ x, uint64_t* __restrict__ y, uint32_t* __restrict__ xkey,
+                          uint32_t* err) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
         const double a = v[u];
         x[u] = a;
-        const uint32_t d = dint[u];
-        y[u] = d ? a / (double)d : 0.0;
+        put_y(y, u, a, dint[u], err);
         xkey[u] = ord_f32((float)a);
     }
 }
@@ -1780,7 +1872,8 @@ int pxd_set_random_walk_values_f64(pxd_engine* e, const double* values) {
         double* tmp = nullptr;
         HIPC(hipMalloc(&tmp, (size_t)e->N * 8));
         HIPC(hipMemcpy(tmp, values, (size_t)e->N * 8, hipMemcpyHostToDevice));
-        k_set_all<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, tmp, e->d_dint, e->d_x[e->cur], e->d_y[e->cur], e->d_xkey);
+        k_set_all<<<grid_for(e->N, BS), BS, 0, e->st>>>(e->N, tmp, e->d_dint, e->d_x[e->cur], e->d_y[e->cur], e->d_xkey,
+                                                         &e->d_cnt->err);
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(e->st));
         HIPC(hipFree(tmp));
@@ -1949,6 +2042,22 @@ int pxd_enable_walk_timing(pxd_engine* e, int enable) {
         e->walk_launches = 0;
         e->walk_ms = 0.0;
         e->walk_bytes = 0.0;
+        e->walk_log_ms.clear();
+        e->walk_log_bytes.clear();
+        return 0;
+    });
+}
+
+int pxd_walk_launch_log(pxd_engine* e, double* ms, double* alg_bytes, uint32_t capacity, uint32_t* count) {
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        const uint32_t n = (uint32_t)e->walk_log_ms.size();
+        if (count) *count = n;
+        if (capacity < n && (ms || alg_bytes)) return set_err(PXD_ERR_ARG, "capacity %u < %u launches", capacity, n);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (ms) ms[i] = e->walk_log_ms[i];
+            if (alg_bytes) alg_bytes[i] = e->walk_log_bytes[i];
+        }
         return 0;
     });
 }
@@ -1959,6 +2068,14 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
         if (launches) *launches = e->walk_launches;
         if (total_ms) *total_ms = e->walk_ms;
         if (alg_bytes) *alg_bytes = e->walk_bytes;
+        return 0;
+    });
+}
+
+int pxd_set_small_graph_kernel(pxd_engine* e, int enable) {
+    return guarded([&]() -> int {
+        if (!e) return set_err(PXD_ERR_ARG, "null engine");
+        e->small_kernel = enable != 0;
         return 0;
     });
 }

@@ -4,7 +4,8 @@
 //   per node u      row[u] (u32 arc offset), deg[u], dint[u] (valid-prefix length),
 //                   nbr[row[u] .. row[u]+dint[u]) = the still-valid arcs of u, kept
 //                   as an order-preserving prefix of the original row;
-//                   x[2][u] (FP64 walk double buffer), y[u] = x[u]/dint[u],
+//                   x[2][u] (FP64 walk double buffer), y[u] = the walk term of
+//                   x[u]/dint[u] packed for exact summation (fix_pack),
 //                   xkey[u] (ordered float of the last walk value), label[u]
 //   per position k  pnix[k] (node), pkey[k] (sort key), pclu[k] (cluster id);
 //                   clusters are contiguous position ranges [cs[c], cs[c+1])
@@ -50,6 +51,14 @@ struct Counters {
 // n from the device when a count pointer is given, else the value passed
 __device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* p) { return p ? *p : n; }
 
+// y[u] = the packed walk term of x / d (0 for d = 0); a value outside the
+// exact fixed-point range sets the error bit of the engine's status word
+__device__ __forceinline__ void put_y(uint64_t* __restrict__ y, uint32_t u, double x, uint32_t d, uint32_t* err) {
+    uint32_t ovf = 0;
+    y[u] = fix_pack(d ? x / (double)d : 0.0, ovf);
+    if (ovf) atomicOr(err, ERR_FIX_RANGE);
+}
+
 // ------------------------------------------------------------------ init
 
 __device__ float init_value(uint32_t idx, uint32_t deg, uint32_t seed) {
@@ -83,12 +92,12 @@ struct ClusterCols {
 __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict__ gnode,
                              const uint32_t* __restrict__ row, uint32_t seed,
                              uint32_t* __restrict__ deg, uint32_t* __restrict__ dint,
-                             double* __restrict__ x, double* __restrict__ y,
+                             double* __restrict__ x, uint64_t* __restrict__ y,
                              uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
                              uint8_t* __restrict__ alive, uint32_t* __restrict__ stamp,
                              uint32_t* __restrict__ pnix, uint32_t* __restrict__ alist,
                              uint32_t* __restrict__ pclu, const uint32_t* __restrict__ gclu,
-                             uint32_t* __restrict__ ngraph) {
+                             uint32_t* __restrict__ ngraph, uint32_t* err) {
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N; u += gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = G;                  // graph of u: last g with gnode[g] <= u
         while (hi - lo > 1) {
@@ -100,7 +109,7 @@ __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict_
         dint[u] = d;
         double v = (double)init_value(u - gnode[lo], d, seed);
         x[u] = v;
-        y[u] = d ? v / (double)d : 0.0;
+        put_y(y, u, v, d, err);
         xkey[u] = ord_f32((float)v);
         label[u] = 0;
         alive[u] = 1;
@@ -119,10 +128,10 @@ __global__ void k_init_nodes(uint32_t N, uint32_t G, const uint32_t* __restrict_
 __global__ void k_admit_nodes(uint32_t base, uint32_t n, uint32_t abase, uint32_t slot, uint32_t c,
                               const uint32_t* __restrict__ rg, uint32_t seed, uint32_t* __restrict__ row,
                               uint32_t* __restrict__ deg, uint32_t* __restrict__ dint, double* __restrict__ x,
-                              double* __restrict__ y, uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
+                              uint64_t* __restrict__ y, uint32_t* __restrict__ xkey, int32_t* __restrict__ label,
                               uint8_t* __restrict__ alive, uint32_t* __restrict__ stamp,
                               uint32_t* __restrict__ ngraph, uint32_t* __restrict__ pnix,
-                              uint32_t* __restrict__ pclu, uint32_t* __restrict__ alist) {
+                              uint32_t* __restrict__ pclu, uint32_t* __restrict__ alist, uint32_t* err) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t u = base + i;
         const uint32_t d = rg[i + 1] - rg[i];
@@ -131,7 +140,7 @@ __global__ void k_admit_nodes(uint32_t base, uint32_t n, uint32_t abase, uint32_
         dint[u] = d;
         const double v = (double)init_value(i, d, seed);
         x[u] = v;
-        y[u] = d ? v / (double)d : 0.0;
+        put_y(y, u, v, d, err);
         xkey[u] = ord_f32((float)v);
         label[u] = 0;
         alive[u] = 1;
@@ -255,13 +264,13 @@ __global__ void k_admit_cluster(uint32_t c, uint32_t na, uint32_t n, uint32_t sl
 // x' = s * (1 - 0.1) + x * 0.1 with s the exact neighbour sum, y' = x'/dInt,
 // walk key (cuda/devRandomWalk.h:186-187)
 __device__ __forceinline__ void walk_finish(uint32_t u, uint32_t d, Fix128 acc, const double* __restrict__ xc,
-                                            double* __restrict__ xn, double* __restrict__ yn,
+                                            double* __restrict__ xn, uint64_t* __restrict__ yn,
                                             uint32_t* __restrict__ xkey, uint32_t& ovf) {
     if (d > (1u << 24)) ovf = 1;
     const double s = fix_to_f64(acc);
     const double v = s * (1.0 - RW_STAY) + xc[u] * RW_STAY;
     xn[u] = v;
-    yn[u] = d ? v / (double)d : 0.0;
+    yn[u] = fix_pack(d ? v / (double)d : 0.0, ovf);
     xkey[u] = ord_f32((float)v);
 }
 
@@ -270,9 +279,9 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
                                              const uint32_t* __restrict__ row,
                                              const uint32_t* __restrict__ dint,
                                              const uint32_t* __restrict__ nbr,
-                                             const double* __restrict__ y,
+                                             const uint64_t* __restrict__ y,
                                              const double* __restrict__ xc, double* __restrict__ xn,
-                                             double* __restrict__ yn, uint32_t* __restrict__ xkey,
+                                             uint64_t* __restrict__ yn, uint32_t* __restrict__ xkey,
                                              uint32_t heavy, Counters* cnt) {
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(na, BS / GS, threadIdx.x / GS);
@@ -282,9 +291,9 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
         const uint32_t start = row[u], d = dint[u];
         if (d > heavy) continue;      // rows above the threshold go to k_walk_heavy
         Fix128 acc = {0ull, 0ull};
-        // four independent index loads, then four value gathers in flight per
+        // four independent index loads, then four term gathers in flight per
         // lane; slots past the row end re-read the row's last arc (no branch,
-        // same line) and add +0.0
+        // same line) and add a zero term
         const uint32_t* p = nbr + start;
         for (uint32_t j = lane; j < d; j += 4 * GS) {
             const bool b1 = j + GS < d, b2 = j + 2 * GS < d, b3 = j + 3 * GS < d;
@@ -292,11 +301,11 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
             const uint32_t v1 = p[min(j + GS, d - 1)];
             const uint32_t v2 = p[min(j + 2 * GS, d - 1)];
             const uint32_t v3 = p[min(j + 3 * GS, d - 1)];
-            const double y0 = y[v0], y1 = y[v1], y2 = y[v2], y3 = y[v3];
-            fix_add_term(acc, y0, ovf);
-            fix_add_term(acc, b1 ? y1 : 0.0, ovf);
-            fix_add_term(acc, b2 ? y2 : 0.0, ovf);
-            fix_add_term(acc, b3 ? y3 : 0.0, ovf);
+            const uint64_t y0 = y[v0], y1 = y[v1], y2 = y[v2], y3 = y[v3];
+            fix_add_packed(acc, y0);
+            fix_add_packed(acc, b1 ? y1 : 0ull);
+            fix_add_packed(acc, b2 ? y2 : 0ull);
+            fix_add_packed(acc, b3 ? y3 : 0ull);
         }
         acc = group_sum_fix<GS>(acc);
         if (lane == 0) walk_finish(u, d, acc, xc, xn, yn, xkey, ovf);
@@ -517,13 +526,13 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
                                                   const uint32_t* __restrict__ nbr,
-                                                  const double* __restrict__ y,
+                                                  const uint64_t* __restrict__ y,
                                                   const double* __restrict__ xc, double* __restrict__ xn,
-                                                  double* __restrict__ yn, uint32_t* __restrict__ xkey,
+                                                  uint64_t* __restrict__ yn, uint32_t* __restrict__ xkey,
                                                   uint32_t heavy, uint32_t N, Counters* cnt,
                                                   const uint32_t* __restrict__ coff = nullptr) {
-    static_assert(WIN % 2 == 0 && 64 % GS == 0, "window of double pairs, lane groups within a wave");
-    __shared__ double sy[WIN];
+    static_assert(WIN % 2 == 0 && 64 % GS == 0, "window of term pairs, lane groups within a wave");
+    __shared__ uint64_t sy[WIN];
     constexpr uint32_t NPR = 64 / GS;           // rows per round
     constexpr uint32_t NW = WBS / 64;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
@@ -558,7 +567,7 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
             wb &= ~1u;
             for (uint32_t k = 2 * threadIdx.x; k < (uint32_t)WIN; k += 2 * WBS) {
                 if (wb + k + 1 < N) {
-                    const double2 t = *(const double2*)(y + wb + k);
+                    const ulonglong2 t = *(const ulonglong2*)(y + wb + k);
                     sy[k] = t.x;
                     sy[k + 1] = t.y;
                 } else if (wb + k < N) {
@@ -584,33 +593,38 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                 Fix128 acc = {0ull, 0ull};
                 for (uint32_t j = gl; j < rd; j += U * GS) {
                     uint32_t q[U];
-                    double yv[U];
+                    uint64_t yv[U];
                     if constexpr (CLAMP) {
 #pragma unroll
                         for (int k = 0; k < U; ++k) q[k] = nbr[rs + min(j + k * GS, rd - 1)];
-#pragma unroll
-                        for (int k = 0; k < U; ++k) {
-                            const uint32_t off = q[k] - wb;
-                            const bool inw = use && off < (uint32_t)WIN;
-                            double t = sy[inw ? off : 0u];
-                            if (!inw) t = y[q[k]];
-                            yv[k] = j + k * GS < rd ? t : 0.0;
-                        }
                     } else {
 #pragma unroll
                         for (int k = 0; k < U; ++k) {
                             const uint32_t jj = j + k * GS;
                             q[k] = jj < rd ? nbr[rs + jj] : 0u;
                         }
+                    }
+                    // every slot reads its window word from LDS; window misses
+                    // then load from global memory.  The barrier keeps the two
+                    // loads apart (the optimizer would otherwise merge them into
+                    // one flat load, which goes through the vector memory path)
+                    bool inw[U];
 #pragma unroll
-                        for (int k = 0; k < U; ++k) {
-                            const uint32_t jj = j + k * GS;
-                            const uint32_t off = q[k] - wb;
-                            yv[k] = jj < rd ? ((use && off < (uint32_t)WIN) ? sy[off] : y[q[k]]) : 0.0;
-                        }
+                    for (int k = 0; k < U; ++k) {
+                        const uint32_t off = q[k] - wb;
+                        inw[k] = use && off < (uint32_t)WIN;
+                        yv[k] = sy[inw[k] ? off : 0u];
                     }
 #pragma unroll
-                    for (int k = 0; k < U; ++k) fix_add_term(acc, yv[k], ovf);
+                    for (int k = 0; k < U; ++k) opaque_u64(yv[k]);
+#pragma unroll
+                    for (int k = 0; k < U; ++k) {
+                        const uint32_t jj = j + k * GS;
+                        if (!inw[k] && (CLAMP || jj < rd)) yv[k] = y[q[k]];
+                        if (jj >= rd) yv[k] = 0ull;
+                    }
+#pragma unroll
+                    for (int k = 0; k < U; ++k) fix_add_packed(acc, yv[k]);
                 }
                 acc = group_sum_fix<GS>(acc);
                 const uint32_t src = (lane % NPR) * GS;
@@ -622,7 +636,7 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                 const double sum = fix_to_f64(res);
                 const double v = sum * (1.0 - RW_STAY) + xo * RW_STAY;
                 xn[u] = v;
-                yn[u] = d ? v / (double)d : 0.0;
+                yn[u] = fix_pack(d ? v / (double)d : 0.0, ovf);
                 xkey[u] = ord_f32((float)v);
             }
         }
@@ -649,7 +663,7 @@ __global__ void __launch_bounds__(BS) k_walk_heavy(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ row,
                                                    const uint32_t* __restrict__ dint,
                                                    const uint32_t* __restrict__ nbr,
-                                                   const double* __restrict__ y, Fix128* __restrict__ part,
+                                                   const uint64_t* __restrict__ y, Fix128* __restrict__ part,
                                                    Counters* cnt) {
     __shared__ Fix128 lds[BS / WAVE];
     const uint32_t H = *nchunks;
@@ -668,12 +682,12 @@ __global__ void __launch_bounds__(BS) k_walk_heavy(const uint32_t* __restrict__ 
             ok[q] = j < d;
             v[q] = p[min(j, d - 1u)];
         }
-        double yv[4];
+        uint64_t yv[4];
 #pragma unroll
         for (uint32_t q = 0; q < 4; ++q) yv[q] = y[v[q]];
         Fix128 acc = {0ull, 0ull};
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) fix_add_term(acc, ok[q] ? yv[q] : 0.0, ovf);
+        for (uint32_t q = 0; q < 4; ++q) fix_add_packed(acc, ok[q] ? yv[q] : 0ull);
         acc = block_sum_fix(acc, lds);
         if (threadIdx.x == 0) part[b] = acc;
         __syncthreads();              // lds reused by the next chunk
@@ -687,7 +701,7 @@ __global__ void k_walk_heavy_finish(const uint32_t* __restrict__ nchunks, const 
                                     const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ alist,
                                     const uint32_t* __restrict__ dint, const Fix128* __restrict__ part,
                                     const double* __restrict__ xc, double* __restrict__ xn,
-                                    double* __restrict__ yn, uint32_t* __restrict__ xkey, Counters* cnt) {
+                                    uint64_t* __restrict__ yn, uint32_t* __restrict__ xkey, Counters* cnt) {
     const uint32_t H = *nchunks;
     uint32_t ovf = 0;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < H; b += gridDim.x * blockDim.x) {
@@ -1158,12 +1172,93 @@ __device__ __forceinline__ void rank_digits(const uint32_t (&dg)[ITEMS], uint32_
     __syncthreads();
 }
 
-// Segmented stable sort of up to SORT_SMALL_N items [segs[0], segs[nseg]) in
-// one workgroup (one launch instead of the size classes and the tile passes):
-// every item gets a label that grows with its segment (the number of segment
-// starts at or before it), then LSD passes over the key digits that vary and
-// over the label digits sort by (label, key); stability keeps key ties in
-// input order.
+// Segmented stable sort of up to NT * ITEMS items [segs[0], segs[nseg]) by one
+// workgroup of NT threads: every item gets a label that grows with its segment
+// (the number of segment starts at or before it), then LSD passes over the key
+// digits that vary and over the label digits sort by (label, key); stability
+// keeps key ties in input order.  Workspace in LDS: buf and lab of NT * ITEMS
+// entries, wcnt[NT / 64][256], hs[256], sred[NT / 64], sbits[2].
+template <int NT, int ITEMS>
+__device__ __forceinline__ void wg_seg_sort(uint32_t nseg, const uint32_t* __restrict__ segs, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ val, uint64_t* buf, uint32_t* lab, uint32_t (*wcnt)[256],
+                            uint32_t* hs, uint32_t* sred, uint32_t* sbits) {
+    constexpr uint32_t CAP = (uint32_t)NT * ITEMS;
+    const uint32_t base = segs[0];
+    const uint32_t cnt = min(segs[nseg] - base, CAP);   // the caller keeps it within the bound
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    for (uint32_t q = threadIdx.x; q < CAP; q += NT) lab[q] = 0u;
+    if (threadIdx.x == 0) { sbits[0] = 0u; sbits[1] = 0xFFFFFFFFu; }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < nseg; c += NT) {
+        const uint32_t st = segs[c] - base;
+        if (st < cnt) atomicAdd(&lab[st], 1u);
+    }
+    __syncthreads();
+    {   // inclusive scan of the start counts, ITEMS consecutive entries a thread
+        const uint32_t q0 = threadIdx.x * ITEMS;
+        uint32_t v[ITEMS], sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < (uint32_t)ITEMS; ++i) { v[i] = lab[q0 + i]; sum += v[i]; }
+        uint32_t tot;
+        uint32_t run = block_excl_scan<NT>(sum, sred, tot);
+#pragma unroll
+        for (uint32_t i = 0; i < (uint32_t)ITEMS; ++i) { run += v[i]; lab[q0 + i] = run; }
+    }
+    __syncthreads();
+    uint64_t e[ITEMS];
+    uint32_t lb[ITEMS];
+    uint32_t lor = 0u, land = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * ITEMS) + (uint32_t)i * WAVE + l;
+        const bool ok = p < cnt;
+        const uint32_t k = ok ? key[base + p] : 0u;
+        e[i] = ((uint64_t)k << 32) | (ok ? val[base + p] : 0u);
+        lb[i] = ok ? lab[p] : 0u;
+        if (ok) { lor |= k; land &= k; }
+    }
+    atomicOr(&sbits[0], lor);
+    atomicAnd(&sbits[1], land);
+    __syncthreads();
+    const uint32_t kbits = sbits[0] ^ sbits[1];               // key bits that vary
+    const uint32_t lbits = nseg > 1u ? 32u - (uint32_t)__clz((int)nseg) : 0u;   // labels are <= nseg
+    for (int pass = 0; pass < 8; ++pass) {
+        const bool on_label = pass >= 4;
+        const int sh = 8 * (pass & 3);
+        if (!on_label && ((kbits >> sh) & 255u) == 0u) continue;   // a constant digit keeps the order
+        if (on_label && (uint32_t)sh >= lbits) break;
+        uint32_t dg[ITEMS], pos[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            dg[i] = on_label ? (lb[i] >> sh) & 255u : (uint32_t)(e[i] >> (32 + sh)) & 255u;
+        rank_digits<NT, ITEMS>(dg, cnt, wcnt, hs, sred, pos);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            if (w * (WAVE * ITEMS) + (uint32_t)i * WAVE + l < cnt) {
+                buf[pos[i]] = e[i];
+                lab[pos[i]] = lb[i];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t p = w * (WAVE * ITEMS) + (uint32_t)i * WAVE + l;
+            if (p < cnt) { e[i] = buf[p]; lb[i] = lab[p]; }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t p = w * (WAVE * ITEMS) + (uint32_t)i * WAVE + l;
+        if (p < cnt) {
+            key[base + p] = (uint32_t)(e[i] >> 32);
+            val[base + p] = (uint32_t)e[i];
+        }
+    }
+    __syncthreads();
+}
+
+// The segmented sort of up to SORT_SMALL_N items in one launch (instead of
+// the size classes and the tile passes).
 constexpr uint32_t SS_NT = 512, SS_ITEMS = 8;
 constexpr uint32_t SORT_SMALL_N = SS_NT * SS_ITEMS;
 constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;   // sorts of up to this many items take k_sort_small
@@ -1175,78 +1270,8 @@ __global__ void __launch_bounds__(SS_NT) k_sort_small(uint32_t nseg, const uint3
     __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t hs[256];
     __shared__ uint32_t sred[NW];
-    __shared__ uint32_t s_or, s_and;
-    const uint32_t base = segs[0];
-    const uint32_t cnt = min(segs[nseg] - base, SORT_SMALL_N);   // the host keeps it within the bound
-    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
-    for (uint32_t q = threadIdx.x; q < SORT_SMALL_N; q += SS_NT) lab[q] = 0u;
-    if (threadIdx.x == 0) { s_or = 0u; s_and = 0xFFFFFFFFu; }
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < nseg; c += SS_NT) {
-        const uint32_t st = segs[c] - base;
-        if (st < cnt) atomicAdd(&lab[st], 1u);
-    }
-    __syncthreads();
-    {   // inclusive scan of the start counts, SS_ITEMS consecutive entries a thread
-        const uint32_t q0 = threadIdx.x * SS_ITEMS;
-        uint32_t v[SS_ITEMS], sum = 0;
-#pragma unroll
-        for (uint32_t i = 0; i < SS_ITEMS; ++i) { v[i] = lab[q0 + i]; sum += v[i]; }
-        uint32_t tot;
-        uint32_t run = block_excl_scan<SS_NT>(sum, sred, tot);
-#pragma unroll
-        for (uint32_t i = 0; i < SS_ITEMS; ++i) { run += v[i]; lab[q0 + i] = run; }
-    }
-    __syncthreads();
-    uint64_t e[SS_ITEMS];
-    uint32_t lb[SS_ITEMS];
-    uint32_t lor = 0u, land = 0xFFFFFFFFu;
-#pragma unroll
-    for (int i = 0; i < (int)SS_ITEMS; ++i) {
-        const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
-        const bool ok = p < cnt;
-        const uint32_t k = ok ? key[base + p] : 0u;
-        e[i] = ((uint64_t)k << 32) | (ok ? val[base + p] : 0u);
-        lb[i] = ok ? lab[p] : 0u;
-        if (ok) { lor |= k; land &= k; }
-    }
-    atomicOr(&s_or, lor);
-    atomicAnd(&s_and, land);
-    __syncthreads();
-    const uint32_t kbits = s_or ^ s_and;                      // key bits that vary
-    const uint32_t lbits = nseg > 1u ? 32u - (uint32_t)__clz((int)nseg) : 0u;   // labels are <= nseg
-    for (int pass = 0; pass < 8; ++pass) {
-        const bool on_label = pass >= 4;
-        const int sh = 8 * (pass & 3);
-        if (!on_label && ((kbits >> sh) & 255u) == 0u) continue;   // a constant digit keeps the order
-        if (on_label && (uint32_t)sh >= lbits) break;
-        uint32_t dg[SS_ITEMS], pos[SS_ITEMS];
-#pragma unroll
-        for (int i = 0; i < (int)SS_ITEMS; ++i)
-            dg[i] = on_label ? (lb[i] >> sh) & 255u : (uint32_t)(e[i] >> (32 + sh)) & 255u;
-        rank_digits<SS_NT, SS_ITEMS>(dg, cnt, wcnt, hs, sred, pos);
-#pragma unroll
-        for (int i = 0; i < (int)SS_ITEMS; ++i)
-            if (w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l < cnt) {
-                buf[pos[i]] = e[i];
-                lab[pos[i]] = lb[i];
-            }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < (int)SS_ITEMS; ++i) {
-            const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
-            if (p < cnt) { e[i] = buf[p]; lb[i] = lab[p]; }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < (int)SS_ITEMS; ++i) {
-        const uint32_t p = w * (WAVE * SS_ITEMS) + (uint32_t)i * WAVE + l;
-        if (p < cnt) {
-            key[base + p] = (uint32_t)(e[i] >> 32);
-            val[base + p] = (uint32_t)e[i];
-        }
-    }
+    __shared__ uint32_t sbits[2];
+    wg_seg_sort<SS_NT, SS_ITEMS>(nseg, segs, key, val, buf, lab, wcnt, hs, sred, sbits);
 }
 
 // Large segments (more than TILE items): LSD radix sort over the four 8-bit
@@ -2091,8 +2116,8 @@ __global__ void k_node_cluster(uint32_t na, const uint32_t* __restrict__ pnix, c
 __global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ nclu,
                               const uint32_t* __restrict__ ckeep, const int32_t* __restrict__ clabel,
                               const double* __restrict__ cavg, const uint32_t* __restrict__ dint,
-                              int32_t* __restrict__ label, double* __restrict__ x, double* __restrict__ y,
-                              uint8_t* __restrict__ alive) {
+                              int32_t* __restrict__ label, double* __restrict__ x, uint64_t* __restrict__ y,
+                              uint8_t* __restrict__ alive, uint32_t* err) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t u = alist[i];
         const uint32_t c = nclu[u];
@@ -2100,8 +2125,7 @@ __global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, co
         if (ckeep[c]) {
             const double v = x[u] - cavg[c];
             x[u] = v;
-            const uint32_t d = dint[u];
-            y[u] = d ? v / (double)d : 0.0;
+            put_y(y, u, v, dint[u], err);
         } else {
             alive[u] = 0;
         }
@@ -2161,8 +2185,8 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
                                                 const uint32_t* __restrict__ row,
                                                 uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                 const uint32_t* __restrict__ root,
-                                                const double* __restrict__ x, double* __restrict__ y,
-                                                uint32_t heavy, const uint32_t* pn = nullptr) {
+                                                const double* __restrict__ x, uint64_t* __restrict__ y,
+                                                uint32_t heavy, uint32_t* err, const uint32_t* pn = nullptr) {
     nl = dev_count(nl, pn);
     const uint32_t lane = threadIdx.x & (GS - 1);
     const XcdSpan sp = xcd_span(nl, BS / GS, threadIdx.x / GS);
@@ -2194,7 +2218,7 @@ __global__ void __launch_bounds__(BS) k_disable(uint32_t nl, const uint32_t* __r
         }
         if (lane == 0 && kept != d) {
             dint[u] = kept;
-            y[u] = kept ? x[u] / (double)kept : 0.0;
+            put_y(y, u, x[u], kept, err);
         }
     }
 }
@@ -2212,8 +2236,9 @@ __global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t
                                                      const uint32_t* __restrict__ row,
                                                      uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ root,
-                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                     uint32_t heavy, uint32_t N, const uint32_t* pn = nullptr) {
+                                                     const double* __restrict__ x, uint64_t* __restrict__ y,
+                                                     uint32_t heavy, uint32_t N, uint32_t* err,
+                                                     const uint32_t* pn = nullptr) {
     static_assert(WIN % 4 == 0 && 64 % GS == 0, "window of root quads, lane groups within a wave");
     __shared__ uint32_t sr[WIN];
     nl = dev_count(nl, pn);
@@ -2293,7 +2318,7 @@ __global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t
             }
             if (mine && res != d) {
                 dint[u] = res;
-                y[u] = res ? x[u] / (double)res : 0.0;
+                put_y(y, u, x[u], res, err);
             }
         }
         __syncthreads();
@@ -2320,7 +2345,8 @@ __global__ void __launch_bounds__(BS) k_disable_heavy(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ row,
                                                       uint32_t* __restrict__ dint, uint32_t* __restrict__ nbr,
                                                       const uint32_t* __restrict__ root,
-                                                      const double* __restrict__ x, double* __restrict__ y) {
+                                                      const double* __restrict__ x, uint64_t* __restrict__ y,
+                                                      uint32_t* err) {
     __shared__ uint32_t lds[BS / WAVE];
     const uint32_t R = *nrows;
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
@@ -2347,7 +2373,7 @@ __global__ void __launch_bounds__(BS) k_disable_heavy(const uint32_t* __restrict
         }
         if (threadIdx.x == 0 && kept != d) {
             dint[u] = kept;
-            y[u] = kept ? x[u] / (double)kept : 0.0;
+            put_y(y, u, x[u], kept, err);
         }
         __syncthreads();
     }

@@ -2,8 +2,9 @@
 
 The engine source is compiled unchanged except that every kernel launch
 `k<<<grid, block, shmem, stream>>>(args)` is rewritten into a call of the
-emulator's launcher, and that the scans' and sorts' size thresholds are
-lowered so that small graphs also take the multi-tile paths.  Output: tests/emu/_build/libpxd_emu.so (and
+emulator's launcher, that the scans' and sorts' size thresholds are
+lowered so that small graphs also take the multi-tile paths, and that the
+device-only optimizer barrier is dropped.  Output: tests/emu/_build/libpxd_emu.so (and
 libpxd_emu_asan.so with AddressSanitizer when asan=True).
 """
 import os
@@ -69,7 +70,7 @@ def build(asan=False):
     open(gen, "w").write(transform(src))
     # headers with launches of their own: transformed copies next to the
     # generated source, which its quoted includes find first
-    for h in ("pxd_ingest.h",):
+    for h in ("pxd_ingest.h", "pxd_small.h"):
         open(os.path.join(OUT, h), "w").write(transform(open(os.path.join(CSRC, h)).read()))
     # the kernel header with small look-back scan tiles (1024 elements), no
     # one-workgroup scans above 1024 elements and one-workgroup sorts only up
@@ -82,6 +83,12 @@ def build(asan=False):
         assert kh.count(old) == 1, old
         kh = kh.replace(old, new)
     open(os.path.join(OUT, "pxd_kernels.h"), "w").write(kh)
+    # the device helpers without the VGPR constraint of the optimizer barrier,
+    # which the host compiler does not know (the emulation needs no barrier)
+    dh = open(os.path.join(CSRC, "pxd_device.h")).read()
+    old = '__asm__ volatile("" : "+v"(v));'
+    assert dh.count(old) == 1, old
+    open(os.path.join(OUT, "pxd_device.h"), "w").write(dh.replace(old, "(void)v;"))
     lib = os.path.join(OUT, "libpxd_emu_asan.so" if asan else "libpxd_emu.so")
     flags = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address"] if asan else ["-O2"]
     cmd = ["g++", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", *flags, "-pthread",

@@ -30,6 +30,7 @@
 
 struct emu_dim3 { uint32_t x = 0, y = 1, z = 1; };
 struct double2 { double x, y; };
+struct ulonglong2 { unsigned long long x, y; };
 struct uint4 { uint32_t x, y, z, w; };
 struct uint2 { uint32_t x, y; };
 inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
@@ -129,6 +130,10 @@ template <class T> inline T atomicMin(T* p, T v) {
 #define __HIP_MEMORY_SCOPE_AGENT 3
 template <class T> inline T __hip_atomic_load(T* p, int order, int) { (void)order; return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 template <class T> inline void __hip_atomic_store(T* p, T v, int order, int) { (void)order; __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
+template <class T> inline T atomicCAS(T* p, T cmp, T v) {
+    __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return cmp;                       // the old value (cmp when the exchange happened)
+}
 template <class T> inline T atomicMax(T* p, T v) {
     T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
     while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}

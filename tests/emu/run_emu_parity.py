@@ -32,6 +32,12 @@ GRAPHS = {
     # 2500 positions: the CC regroup sorts one segment of more than a tile
     # (the large-segment plan built on the device, several tiles)
     "sbm2500": lambda: px.sbm(2500, 2, 0.01, 0.0004, 7),
+    # triangles, a path and isolated nodes; a graph without arcs
+    "islands": lambda: px.from_edges(40, [0, 1, 2, 5, 6, 7, 20, 21, 22, 22], [1, 2, 0, 6, 7, 5, 21, 22, 23, 24]),
+    "noarcs": lambda: px.from_edges(30, [], []),
+    # one row of 599 arcs on a ring: rows above the one-thread limit of the
+    # one-workgroup run
+    "hub600": lambda: px.from_edges(*_hub_edges(600)),
 }
 
 
@@ -82,11 +88,20 @@ def main(lib_path, names):
             if not same:
                 return 1
             continue
-        g = GRAPHS[name]()
+        # "graph@v": the per-arc kernels pinned to variant v (pxd_set_walk_variant);
+        # "graph/phases": the per-phase kernels instead of the one-workgroup run
+        # that a small single graph takes by default
+        name, _, mode = name.partition("/")
+        gname, _, var = name.partition("@")
+        g = GRAPHS[gname]()
         p = Params()
         L.pxd_default_params(C.byref(p))
         h = C.c_void_p()
         chk(L.pxd_create(C.byref(p), C.byref(h)))
+        if var:
+            chk(L.pxd_set_walk_variant(h, int(var)))
+        if mode == "phases":
+            chk(L.pxd_set_small_graph_kernel(h, 0))
         r, e = np.ascontiguousarray(g.ranges, np.uint32), np.ascontiguousarray(g.edges, np.uint32)
         chk(L.pxd_initialize(h, g.n, g.m, ptr(r), ptr(e)))
         x0 = np.zeros(g.n)
