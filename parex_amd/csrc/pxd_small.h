@@ -27,7 +27,8 @@ constexpr int SM_NT = 1024;                          // threads of the workgroup
 constexpr int SM_ITEMS = 4;                          // items per thread in tile passes
 constexpr uint32_t SMALL_MAX_N = (uint32_t)SM_NT * SM_ITEMS;   // 4096 nodes
 constexpr uint32_t SMALL_MAX_ARCS = 1u << 19;        // beyond this the multi-kernel walk wins
-constexpr uint32_t SM_LONG = 32;                     // longer rows: one wave each, else one thread
+constexpr uint32_t SM_LONG = 32;                     // longer rows: 16 lanes each, else one thread
+constexpr uint32_t SM_GL = 16;                       // lanes per long row: four long rows a wave
 constexpr uint32_t SM_NW = SM_NT / WAVE;
 
 struct SmallArgs {
@@ -82,6 +83,22 @@ __device__ uint32_t sm_compact(uint32_t* a, uint32_t n, K&& keep, uint32_t* sred
         if (f & (1u << q)) a[o++] = v[q];
     __syncthreads();
     return tot;
+}
+
+// The long rows among a wave's 64 entries (set lanes of the wave-uniform
+// mask m) go four at a time to the wave's four 16-lane groups: group g takes
+// the g-th lowest set lane.  Returns that lane, or 64 when the group has no
+// row this round, and clears the taken lanes from m.
+__device__ __forceinline__ uint32_t sm_take4(uint64_t& m) {
+    const uint32_t g = lane_id() / SM_GL;
+    uint32_t src = WAVE;
+#pragma unroll
+    for (uint32_t q = 0; q < WAVE / SM_GL; ++q) {
+        const uint32_t r = m ? (uint32_t)__builtin_ctzll(m) : (uint32_t)WAVE;
+        if (q == g) src = r;
+        if (m) m &= m - 1ull;
+    }
+    return src;
 }
 
 // union-find in LDS: parents only ever point to smaller nodes of the same
@@ -164,18 +181,21 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
                 }
                 walk_finish(u, d, acc, xc, xn, yn, a.xkey, ovf);
             }
-            // longer rows: one wave each
+            // longer rows: 16 lanes each, four rows of a wave at a time
             for (uint32_t b = w * WAVE; b < nalist; b += SM_NT) {
                 const uint32_t e = b + l;
                 const uint32_t u = e < nalist ? alist[e] : 0u;
                 const uint32_t d = e < nalist ? a.dint[u] : 0u;
-                for (uint64_t m = __ballot(d > SM_LONG); m; m &= m - 1ull) {
-                    const int r = __builtin_ctzll(m);
-                    const uint32_t ur = __shfl(u, r), dr = __shfl(d, r), st = a.row[ur];
-                    Fix128 acc = {0ull, 0ull};
-                    for (uint32_t j = l; j < dr; j += WAVE) fix_add_packed(acc, yc[a.nbr[st + j]]);
-                    acc = group_sum_fix<WAVE>(acc);
-                    if (l == 0) walk_finish(ur, dr, acc, xc, xn, yn, a.xkey, ovf);
+                for (uint64_t m = __ballot(d > SM_LONG); m;) {
+                    const uint32_t src = sm_take4(m);
+                    const uint32_t ur = __shfl(u, (int)(src & 63u)), dr = __shfl(d, (int)(src & 63u));
+                    if (src < (uint32_t)WAVE) {            // uniform within the 16-lane group
+                        const uint32_t st = a.row[ur], gl = l % SM_GL;
+                        Fix128 acc = {0ull, 0ull};
+                        for (uint32_t j = gl; j < dr; j += SM_GL) fix_add_packed(acc, yc[a.nbr[st + j]]);
+                        acc = group_sum_fix<SM_GL>(acc);
+                        if (gl == 0) walk_finish(ur, dr, acc, xc, xn, yn, a.xkey, ovf);
+                    }
                 }
             }
             cur ^= 1u;
@@ -210,16 +230,19 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
             const uint32_t e = b + l;
             const uint32_t u = e < nalist ? alist[e] : 0u;
             const uint32_t d = e < nalist ? a.dint[u] : 0u;
-            for (uint64_t m = __ballot(d > SM_LONG); m; m &= m - 1ull) {
-                const int r = __builtin_ctzll(m);
-                const uint32_t ur = __shfl(u, r), dr = __shfl(d, r), st = a.row[ur], mk = a.xkey[ur];
-                int32_t c = 0;
-                for (uint32_t j = l; j < dr; j += WAVE) {
-                    const uint32_t v = a.nbr[st + j], kv = a.xkey[v];
-                    c += ((kv != mk) ? (kv < mk) : (v < ur)) ? -1 : 1;
+            for (uint64_t m = __ballot(d > SM_LONG); m;) {
+                const uint32_t src = sm_take4(m);
+                const uint32_t ur = __shfl(u, (int)(src & 63u)), dr = __shfl(d, (int)(src & 63u));
+                if (src < (uint32_t)WAVE) {                // uniform within the 16-lane group
+                    const uint32_t st = a.row[ur], mk = a.xkey[ur], gl = l % SM_GL;
+                    int32_t c = 0;
+                    for (uint32_t j = gl; j < dr; j += SM_GL) {
+                        const uint32_t v = a.nbr[st + j], kv = a.xkey[v];
+                        c += ((kv != mk) ? (kv < mk) : (v < ur)) ? -1 : 1;
+                    }
+                    c = group_sum<SM_GL>(c);
+                    if (gl == 0) a.edv[ur] = make_uint2((uint32_t)c, dr);
                 }
-                c = group_sum<WAVE>(c);
-                if (l == 0) a.edv[ur] = make_uint2((uint32_t)c, dr);
             }
         }
         unsigned long long* s_best = reinterpret_cast<unsigned long long*>(s_buf);   // per cluster: best cut
@@ -340,12 +363,15 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
                 const uint32_t e = b + l;
                 const uint32_t u = e < nl ? a.cclist[e] : 0u;
                 const uint32_t d = e < nl ? a.dint[u] : 0u;
-                for (uint64_t m = __ballot(d > SM_LONG); m; m &= m - 1ull) {
-                    const int r = __builtin_ctzll(m);
-                    const uint32_t ur = __shfl(u, r), dr = __shfl(d, r), st = a.row[ur], pu = s_part[ur];
-                    for (uint32_t j = l; j < dr; j += WAVE) {
-                        const uint32_t v = a.nbr[st + j];
-                        if (s_part[v] == pu) sm_union(s_root, ur, v);
+                for (uint64_t m = __ballot(d > SM_LONG); m;) {
+                    const uint32_t src = sm_take4(m);
+                    const uint32_t ur = __shfl(u, (int)(src & 63u)), dr = __shfl(d, (int)(src & 63u));
+                    if (src < (uint32_t)WAVE) {
+                        const uint32_t st = a.row[ur], pu = s_part[ur];
+                        for (uint32_t j = l % SM_GL; j < dr; j += SM_GL) {
+                            const uint32_t v = a.nbr[st + j];
+                            if (s_part[v] == pu) sm_union(s_root, ur, v);
+                        }
                     }
                 }
             }
@@ -582,28 +608,31 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
                     a.y[cur][u] = fix_pack(kept ? a.x[cur][u] / (double)kept : 0.0, ovf);
                 }
             }
-            // long rows: one wave each, 64 arcs at a time ranked by a ballot
+            // long rows: 16 lanes each, 16 arcs at a time ranked by a group ballot
             for (uint32_t b = w * WAVE; b < nl; b += SM_NT) {
                 const uint32_t e = b + l;
                 const uint32_t u = e < nl ? a.cclist[e] : 0u;
                 const uint32_t d = e < nl ? a.dint[u] : 0u;
                 const bool mine = e < nl && a.alive[u] && d > SM_LONG;
-                for (uint64_t m = __ballot(mine); m; m &= m - 1ull) {
-                    const int r = __builtin_ctzll(m);
-                    const uint32_t ur = __shfl(u, r), dr = __shfl(d, r), st = a.row[ur], ru = a.root[ur];
-                    uint32_t kept = 0;
-                    for (uint32_t j0 = 0; j0 < dr; j0 += WAVE) {
-                        const uint32_t j = j0 + l;
-                        const uint32_t v = a.nbr[st + min(j, dr - 1u)];
-                        const bool keep = j < dr && a.root[v] == ru;
-                        const uint64_t bal = __ballot(keep);
-                        const uint32_t before = (uint32_t)__popcll(bal & ((1ull << l) - 1ull));
-                        if (keep && kept + before != j) a.nbr[st + kept + before] = v;
-                        kept += (uint32_t)__popcll(bal);
-                    }
-                    if (l == 0 && kept != dr) {
-                        a.dint[ur] = kept;
-                        a.y[cur][ur] = fix_pack(kept ? a.x[cur][ur] / (double)kept : 0.0, ovf);
+                for (uint64_t m = __ballot(mine); m;) {
+                    const uint32_t src = sm_take4(m);
+                    const uint32_t ur = __shfl(u, (int)(src & 63u)), dr = __shfl(d, (int)(src & 63u));
+                    if (src < (uint32_t)WAVE) {            // uniform within the 16-lane group
+                        const uint32_t st = a.row[ur], ru = a.root[ur], gl = l % SM_GL;
+                        uint32_t kept = 0;
+                        for (uint32_t j0 = 0; j0 < dr; j0 += SM_GL) {
+                            const uint32_t j = j0 + gl;
+                            const uint32_t v = a.nbr[st + min(j, dr - 1u)];
+                            const bool keep = j < dr && a.root[v] == ru;
+                            const uint64_t bal = group_ballot<SM_GL>(keep);
+                            const uint32_t before = (uint32_t)__popcll(bal & ((1ull << gl) - 1ull));
+                            if (keep && kept + before != j) a.nbr[st + kept + before] = v;
+                            kept += (uint32_t)__popcll(bal);
+                        }
+                        if (gl == 0 && kept != dr) {
+                            a.dint[ur] = kept;
+                            a.y[cur][ur] = fix_pack(kept ? a.x[cur][ur] / (double)kept : 0.0, ovf);
+                        }
                     }
                 }
             }

@@ -38,6 +38,10 @@ GRAPHS = {
     # one row of 599 arcs on a ring: rows above the one-thread limit of the
     # one-workgroup run
     "hub600": lambda: px.from_edges(*_hub_edges(600)),
+    # every row above the one-thread limit: the 16-lane groups of the
+    # one-workgroup run, several rows per wave
+    "clique48": lambda: px.from_edges(48, *np.triu_indices(48, 1)),
+    "sbm_dense": lambda: px.sbm(400, 4, 0.4, 0.02, 5),
 }
 
 
