@@ -409,6 +409,8 @@ struct pxd_engine {
         CHK(alloc(&d_hscan, (size_t)tiles_cap * 256 + 2));
         // the largest scans: N + 1 elements, or the tile histograms
         lb_scan_cap = cdiv(std::max<uint64_t>((uint64_t)n1 + 1, (uint64_t)tiles_cap * 256 + 1), LB_SCAN_TILE) + 1;
+        // the selections' short tiles (lists shorter than SE_LONG)
+        lb_scan_cap = std::max<uint32_t>(lb_scan_cap, cdiv(n1, BS * SE_ROWS_SHORT) + 1);
         CHK(alloc(&d_lb_scan, lb_scan_cap));
         CHK(alloc(&d_lb_ticket, 1));
         const uint32_t nbk = cdiv(n1, BS * SCAN_ITEMS) + 1;
@@ -667,6 +669,21 @@ struct pxd_engine {
             CHK(next_epoch(&ep));
             k_scan_lb<<<nb, BS, 0, st>>>(n, in, out, d_lb_scan, d_lb_ticket, ep, pn, total);
         }
+        HIPC(hipGetLastError());
+        return 0;
+    }
+
+    // one launch of k_scan_emit over n elements (n = *pn on the device when pn
+    // is given; n bounds it and sizes the grid); the flagged count to *total
+    template <class F>
+    int select(uint32_t n, const F& f, const uint32_t* pn = nullptr, uint32_t* total = nullptr) {
+        const bool lng = n >= SE_LONG;
+        const uint32_t nb = std::max(cdiv(n, BS * (lng ? SE_ROWS : SE_ROWS_SHORT)), 1u);
+        if (nb > lb_scan_cap) return set_err(PXD_ERR_STATE, "selection over %u elements exceeds the status capacity", n);
+        uint32_t ep;
+        CHK(next_epoch(&ep));
+        if (lng) k_scan_emit<SE_ROWS, F><<<nb, BS, 0, st>>>(n, f, d_lb_scan, d_lb_ticket, ep, pn, total);
+        else k_scan_emit<SE_ROWS_SHORT, F><<<nb, BS, 0, st>>>(n, f, d_lb_scan, d_lb_ticket, ep, pn, total);
         HIPC(hipGetLastError());
         return 0;
     }
@@ -1010,23 +1027,16 @@ struct pxd_engine {
                                                     d_root, d_stamp);
             // the CC list in node order (the active list holds the positions' nodes);
             // its length is the member count the sweep returned
-            k_member_flags<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_stamp, iter1, d_tA);
             HIPC(hipGetLastError());
-            CHK(scan(d_tA, d_tB, nalist));
-            k_compact_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tA, d_tB, d_alist[acur], d_cclist);
-            HIPC(hipGetLastError());
+            CHK(select(nalist, SelMembers{d_alist[acur], d_stamp, iter1, d_cclist}));
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
             k_one_segment<<<1, 64, 0, st>>>(d_oneseg, na, nullptr);
             HIPC(hipGetLastError());
             CHK(seg_sort(d_oneseg, 1, na));
-            k_run_flags<<<grid_for(na, BS), BS, 0, st>>>(na, d_pkey[0], d_tA);
-            HIPC(hipGetLastError());
-            CHK(scan(d_tA, d_tB, na, nullptr, &d_cnt->ncr));
-            k_build_clusters<<<grid_for(na, BS), BS, 0, st>>>(na, d_ngraph, d_pkey[0], d_tA, d_tB, T->cs, T->cmin,
-                                                              T->cgraph, T->cflag, d_pclu[0]);
-            HIPC(hipGetLastError());
+            CHK(select(na, SelRuns{na, d_pkey[0], d_ngraph, T->cs, T->cmin, T->cgraph, T->cflag, d_pclu[0]}, nullptr,
+                       &d_cnt->ncr));
             // components of the CC clusters are at most their members
             ncb = (uint32_t)std::min<uint64_t>(na, (uint64_t)nc + nl);
             pnc = &d_cnt->ncr;
@@ -1047,15 +1057,14 @@ struct pxd_engine {
         k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter,
                                                        d_gstart, iter1);
         // the active node list holds the same nodes as the positions: apply in index order
-        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_ckeep, d_nclu, d_tA);
+        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_nclu);
         k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
                                                            d_dint, d_label, d_x[cur], d_y[cur], d_alive, &d_cnt->err);
         HIPC(hipGetLastError());
-        // drop deactivated clusters from the position list, the cluster table and the node list
-        // (keep flags per position from k_node_cluster)
-        CHK(scan(d_tA, d_tB, na, nullptr, &d_cnt->na_new));
-        k_pos_compact<<<grid_for(na, BS), BS, 0, st>>>(na, d_tA, d_tB, d_pnix[0], d_pclu[0], d_kscan, d_pnix[1],
-                                                       d_pclu[1]);
+        // drop deactivated clusters from the position list, the cluster table and
+        // the node list (d_tB: every kept cluster's new first position)
+        CHK(select(na, SelPositions{d_pnix[0], d_pclu[0], d_ckeep, d_kscan, T->cs, d_pnix[1], d_pclu[1], d_tB},
+                   nullptr, &d_cnt->na_new));
         ClusterTable& T2 = ct[ctc ^ 1];
         k_clu_compact<<<gc, BS, 0, st>>>(ncb, 0u, d_ckeep, d_kscan, d_tB, T->cols(), T2.cols(), pnc, &d_cnt->na_new);
         HIPC(hipGetLastError());
@@ -1069,10 +1078,7 @@ struct pxd_engine {
         std::swap(d_pclu[0], d_pclu[1]);
         ctc ^= 1;
         if (na_new != na) {
-            k_alive_flags<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_alive, d_tA);
-            CHK(scan(d_tA, d_tC, nalist));
-            k_compact_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tA, d_tC, d_alist[acur], d_alist[acur ^ 1]);
-            HIPC(hipGetLastError());
+            CHK(select(nalist, SelAlive{d_alist[acur], d_alive, d_alist[acur ^ 1]}));
             acur ^= 1;
             nalist = na_new;
         }
@@ -1134,13 +1140,11 @@ struct pxd_engine {
             }
             k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
         }
-        k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant);
-        k_cc_offgiant<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_part, d_giant, d_root, d_tC, d_cnt);
+        k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant, d_cnt);
         HIPC(hipGetLastError());
-        CHK(scan(d_tC, d_tA, nl));
-        const uint32_t* pnu = d_tA + nl;      // members outside the giant trees (<= nl)
-        k_compact_u32<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_tC, d_tA, d_cclist, d_cclist2);   // over all nl flags
-        HIPC(hipGetLastError());
+        // the members outside the giant trees, in list order; their count stays on the device
+        const uint32_t* pnu = &d_cnt->nu;
+        CHK(select(nl, SelOffGiant{d_cclist, d_part, d_giant, d_root, d_cclist2}, nullptr, &d_cnt->nu));
         const bool heavy = heavy_thr != 0xFFFFFFFFu;
         if (heavy) {                  // chunk map of the list's long rows (the walk's map is not needed now)
             heavy_valid = false;

@@ -956,6 +956,158 @@ __global__ void __launch_bounds__(BS) k_scan_lb(uint32_t n, const uint32_t* __re
     }
 }
 
+// Single-pass ranked selection, the status protocol of k_scan_lb: element i
+// of [0, n) has a flag f.flag(i), and every element gets f.emit(i, r, flag)
+// with r = the number of flagged elements before it; the flagged count goes
+// to *total (when given).  One launch replaces a flag kernel, a scan and a
+// scatter kernel (the flags and prefixes never go to memory).  A tile is
+// ROWS rows of BS consecutive elements, row q read by all threads (element
+// base + q * BS + t), so the predicate's loads coalesce; an element's rank
+// inside the tile is the flagged count of the earlier (row, wave) pairs plus
+// the wave ballot's count of flagged lanes below it.  Long lists take tiles
+// of SE_ROWS rows (few tiles for the look-back to cross), shorter ones
+// SE_ROWS_SHORT rows (enough workgroups to occupy the chip).
+constexpr uint32_t SE_ROWS = 32, SE_ROWS_SHORT = 8;
+constexpr uint32_t SE_LONG = 1u << 23;             // lists of at least this many elements: SE_ROWS
+template <uint32_t SE_ROWS, class F>
+__global__ void __launch_bounds__(BS) k_scan_emit(uint32_t n, F f, unsigned long long* status, uint32_t* ticket,
+                                                  uint32_t epoch, const uint32_t* pn, uint32_t* total) {
+    constexpr uint32_t NW = BS / WAVE;
+    constexpr uint32_t SE_TILE = BS * SE_ROWS;
+    static_assert(SE_ROWS * NW <= BS, "one (row, wave) count a thread");
+    __shared__ uint32_t s_cnt[SE_ROWS * NW];
+    __shared__ uint32_t lds[NW];
+    __shared__ uint32_t s_tile, s_first;
+    if (threadIdx.x == 0) s_tile = lb_ticket(ticket);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    n = dev_count(n, pn);
+    const uint32_t ntiles = n ? (n - 1u) / SE_TILE + 1u : 1u;   // tile 0 writes the total when n == 0
+    if (tile >= ntiles) return;
+    const uint32_t base = tile * SE_TILE;
+    const uint32_t w = threadIdx.x / WAVE, l = lane_id();
+    const uint64_t lt = (1ull << l) - 1ull;
+    // all rows' flags first (their loads in flight together), then the ballots
+    uint32_t fl = 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < SE_ROWS; ++q) {
+        const uint32_t i = base + q * BS + threadIdx.x;
+        if (i < n && f.flag(i)) fl |= 1u << q;
+    }
+    uint32_t rk[SE_ROWS];
+#pragma unroll
+    for (uint32_t q = 0; q < SE_ROWS; ++q) {
+        const uint64_t bal = __ballot((fl >> q) & 1u);
+        rk[q] = (uint32_t)__popcll(bal & lt);
+        if (l == 0) s_cnt[q * NW + w] = (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    uint32_t tot;
+    {
+        const uint32_t v = threadIdx.x < SE_ROWS * NW ? s_cnt[threadIdx.x] : 0u;
+        const uint32_t ex = block_excl_scan<BS>(v, lds, tot);
+        if (threadIdx.x < SE_ROWS * NW) s_cnt[threadIdx.x] = ex;
+    }
+    if (threadIdx.x == 0) lb_publish(&status[tile], lb_word(epoch, tile == 0 ? LB_INC : LB_AGG, tot));
+    uint32_t prefix = 0;
+    if (tile != 0) {
+        // windows of BS earlier tiles; the nearest inclusive one ends the walk
+        for (int64_t top = (int64_t)tile - 1;; top -= BS) {
+            const int64_t j = top - (int64_t)threadIdx.x;
+            unsigned long long wd = 0ull;
+            uint32_t stt = LB_INC;    // tiles before 0 read as an inclusive prefix of 0
+            if (j >= 0) {
+                while ((stt = lb_state(wd = lb_read(&status[j]), epoch)) == 0u) __builtin_amdgcn_s_sleep(1);
+            }
+            if (threadIdx.x == 0) s_first = BS;
+            __syncthreads();
+            if (stt == LB_INC) atomicMin(&s_first, threadIdx.x);
+            __syncthreads();
+            const uint32_t first = s_first;
+            uint32_t part;
+            (void)block_excl_scan<BS>(threadIdx.x <= first && j >= 0 ? (uint32_t)wd : 0u, lds, part);
+            prefix += part;
+            if (first < BS) break;
+        }
+        if (threadIdx.x == 0) lb_publish(&status[tile], lb_word(epoch, LB_INC, prefix + tot));
+    }
+    __syncthreads();                  // s_cnt holds the (row, wave) prefixes
+#pragma unroll
+    for (uint32_t q = 0; q < SE_ROWS; ++q) {
+        const uint32_t i = base + q * BS + threadIdx.x;
+        if (i < n) f.emit(i, prefix + s_cnt[q * NW + w] + rk[q], ((fl >> q) & 1u) != 0u);
+    }
+    if (tile == ntiles - 1u && threadIdx.x == 0 && total) *total = prefix + tot;
+}
+
+// k_scan_emit selections of the iteration:
+// the CC members in node order: active nodes stamped by this cut
+struct SelMembers {
+    const uint32_t* alist;
+    const uint32_t* stamp;
+    uint32_t iter;
+    uint32_t* out;
+    __device__ bool flag(uint32_t i) const { return stamp[alist[i]] == iter; }
+    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = alist[i]; }
+};
+// the members outside their part's giant tree
+struct SelOffGiant {
+    const uint32_t* list;
+    const uint32_t* part;
+    const uint32_t* giant;
+    const uint32_t* root;
+    uint32_t* out;
+    __device__ bool flag(uint32_t i) const;
+    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = list[i]; }
+};
+// the positions of the kept clusters, their new cluster ids,
+// and every kept cluster's new first position (cstart, by old cluster id)
+struct SelPositions {
+    const uint32_t* pnix;
+    const uint32_t* pclu;
+    const uint32_t* ckeep;
+    const uint32_t* kscan;
+    const uint32_t* cs;
+    uint32_t *pnix_o, *pclu_o, *cstart;
+    __device__ bool flag(uint32_t k) const { return ckeep[pclu[k]] != 0u; }
+    __device__ void emit(uint32_t k, uint32_t r, bool f) const {
+        if (!f) return;
+        const uint32_t c = pclu[k];
+        pnix_o[r] = pnix[k];
+        pclu_o[r] = kscan[c];
+        if (cs[c] == k) cstart[c] = r;
+    }
+};
+// the active nodes still alive
+struct SelAlive {
+    const uint32_t* alist;
+    const uint8_t* alive;
+    uint32_t* out;
+    __device__ bool flag(uint32_t i) const { return alive[alist[i]] != 0; }
+    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = alist[i]; }
+};
+// the cluster table from runs of equal regroup keys (sorted by minimum node):
+// a run starts a cluster, every position gets its run's id
+struct SelRuns {
+    uint32_t na;
+    const uint32_t* key;
+    const uint32_t* ngraph;
+    uint32_t *cs, *cmin, *cgraph, *cflag, *pclu;
+    __device__ bool flag(uint32_t k) const { return k == 0 || key[k] != key[k - 1]; }
+    __device__ void emit(uint32_t k, uint32_t r, bool f) const {
+        const uint32_t c = r + (f ? 1u : 0u) - 1u;
+        pclu[k] = c;
+        if (f) {
+            const uint32_t kk = key[k];
+            cs[c] = k;
+            cmin[c] = kk;
+            cgraph[c] = ngraph[kk];
+            cflag[c] = CF_CONN;
+        }
+        if (k == na - 1u) cs[c + 1u] = na;
+    }
+};
+
 // ------------------------------------------------------------------ segmented sort
 
 // Sort classes: size <= 1 nothing, <= 64 one wave, <= TILE one block in LDS,
@@ -1634,13 +1786,6 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
     }
 }
 
-// f[i] = 1 for the nodes of the active list stamped in this iteration
-__global__ void k_member_flags(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ stamp,
-                               uint32_t iter, uint32_t* __restrict__ f) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        f[i] = stamp[alist[i]] == iter ? 1u : 0u;
-}
-
 // Connected components of every CC part (parent pointers `root`, members in
 // `list`).  Invariant of every kernel below: a pointer only ever moves to a
 // smaller node of the same component, so trees stay acyclic and the root of a
@@ -1757,7 +1902,11 @@ __device__ __forceinline__ uint32_t cc_find_stable(const uint32_t* __restrict__ 
 // member (by position).  Members already in that tree skip the exact phase.
 __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
                            const uint32_t* __restrict__ coff, const uint32_t* __restrict__ pnix,
-                           uint32_t* __restrict__ root, uint32_t* __restrict__ giant) {
+                           uint32_t* __restrict__ root, uint32_t* __restrict__ giant, Counters* cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the exact rounds that follow start unconverged
+        cnt->cc_done = 0u;
+        cnt->link_changed = 0u;
+    }
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const uint32_t f = cflag[c];
         if (!(f & CF_CC)) continue;
@@ -1768,18 +1917,10 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
     }
 }
 
-// f[i] = 1 for members outside their part's giant tree
-__global__ void k_cc_offgiant(uint32_t nl, const uint32_t* __restrict__ list, const uint32_t* __restrict__ part,
-                              const uint32_t* __restrict__ giant, uint32_t* __restrict__ root,
-                              uint32_t* __restrict__ f, Counters* cnt) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the exact rounds that follow start unconverged
-        cnt->cc_done = 0u;
-        cnt->link_changed = 0u;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
-        const uint32_t u = list[i];
-        f[i] = cc_find_stable(root, u) != giant[part[u]] ? 1u : 0u;
-    }
+// a member outside its part's giant tree
+__device__ __forceinline__ bool SelOffGiant::flag(uint32_t i) const {
+    const uint32_t u = list[i];
+    return cc_find_stable(root, u) != giant[part[u]];
 }
 
 // Exact rounds over the members outside the giant trees, GS lanes per node:
@@ -1900,31 +2041,6 @@ __global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint
         } else {
             key[k] = cmin[c];
         }
-    }
-}
-
-__global__ void k_run_flags(uint32_t na, const uint32_t* __restrict__ key, uint32_t* __restrict__ f) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
-        f[k] = (k == 0 || key[k] != key[k - 1]) ? 1u : 0u;
-}
-
-// new cluster table from runs of equal regroup keys (sorted by minimum node)
-__global__ void k_build_clusters(uint32_t na, const uint32_t* __restrict__ ngraph,
-                                 const uint32_t* __restrict__ key, const uint32_t* __restrict__ f,
-                                 const uint32_t* __restrict__ fs, uint32_t* __restrict__ cs,
-                                 uint32_t* __restrict__ cmin, uint32_t* __restrict__ cgraph,
-                                 uint32_t* __restrict__ cflag, uint32_t* __restrict__ pclu) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
-        const uint32_t c = fs[k] + f[k] - 1u;
-        pclu[k] = c;
-        if (f[k]) {
-            const uint32_t r = key[k];
-            cs[c] = k;
-            cmin[c] = r;
-            cgraph[c] = ngraph[r];
-            cflag[c] = CF_CONN;
-        }
-        if (k == na - 1) cs[c + 1] = na;
     }
 }
 
@@ -2099,16 +2215,11 @@ __global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive,
     }
 }
 
-// per position: the cluster of its node (nclu[pnix[k]] = pclu[k]) and
-// whether that cluster stays active
+// per position: the cluster of its node (nclu[pnix[k]] = pclu[k])
 __global__ void k_node_cluster(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                               const uint32_t* __restrict__ ckeep, uint32_t* __restrict__ nclu,
-                               uint32_t* __restrict__ f) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
-        const uint32_t c = pclu[k];
-        nclu[pnix[k]] = c;
-        f[k] = ckeep[c];
-    }
+                               uint32_t* __restrict__ nclu) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
+        nclu[pnix[k]] = pclu[k];
 }
 
 // per active node, in index order: write labels, recentre survivors
@@ -2134,16 +2245,9 @@ __global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, co
 
 // ------------------------------------------------------------------ compaction
 
-__global__ void k_pos_compact(uint32_t na, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,
-                              const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                              const uint32_t* __restrict__ cscan, uint32_t* __restrict__ pnix_o,
-                              uint32_t* __restrict__ pclu_o) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
-        if (f[k]) { pnix_o[fs[k]] = pnix[k]; pclu_o[fs[k]] = cscan[pclu[k]]; }
-}
-
+// cstart[c]: the new first position of kept cluster c (SelPositions)
 __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __restrict__ ckeep,
-                              const uint32_t* __restrict__ cscan, const uint32_t* __restrict__ pscan,
+                              const uint32_t* __restrict__ cscan, const uint32_t* __restrict__ cstart,
                               ClusterCols in, ClusterCols out, const uint32_t* pnc = nullptr,
                               const uint32_t* pna_new = nullptr) {
     nc = dev_count(nc, pnc);
@@ -2151,7 +2255,7 @@ __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __re
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         if (!ckeep[c]) continue;
         const uint32_t d = cscan[c];
-        out.cs[d] = pscan[in.cs[c]];
+        out.cs[d] = cstart[c];
         out.cgraph[d] = in.cgraph[c];
         out.cmin[d] = in.cmin[c];
         out.cflag[d] = in.cflag[c] & CF_CONN;
@@ -2159,12 +2263,6 @@ __global__ void k_clu_compact(uint32_t nc, uint32_t na_new, const uint32_t* __re
         out.cvol[d] = 0u;             // k_volume adds the new volumes
         if (d + 1 == cscan[nc]) out.cs[d + 1] = na_new;
     }
-}
-
-__global__ void k_alive_flags(uint32_t n, const uint32_t* __restrict__ alist, const uint8_t* __restrict__ alive,
-                              uint32_t* __restrict__ f) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        f[i] = alive[alist[i]];
 }
 
 __global__ void k_compact_u32(uint32_t n, const uint32_t* __restrict__ f, const uint32_t* __restrict__ fs,

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
             }
         }
         __syncthreads();
-        // compaction into the other buffers (k_pos_compact, k_clu_compact):
+        // compaction into the other buffers (SelPositions, k_clu_compact):
         // positions and clusters of the kept clusters, the alive nodes
         uint32_t na_new;
         {

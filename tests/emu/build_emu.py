@@ -72,14 +72,18 @@ def build(asan=False):
     # generated source, which its quoted includes find first
     for h in ("pxd_ingest.h", "pxd_small.h"):
         open(os.path.join(OUT, h), "w").write(transform(open(os.path.join(CSRC, h)).read()))
-    # the kernel header with small look-back scan tiles (1024 elements), no
-    # one-workgroup scans above 1024 elements and one-workgroup sorts only up
-    # to 1024 items, so that the small graphs of the parity runs already take
-    # the single-pass scan over several tiles and the size-class sort
+    # the kernel header with small look-back scan and selection tiles (1024
+    # elements), no one-workgroup scans above 1024 elements and one-workgroup
+    # sorts only up to 1024 items, so that the small graphs of the parity runs
+    # already take the single-pass scan and selection over several tiles and
+    # the size-class sort
     kh = open(os.path.join(CSRC, "pxd_kernels.h")).read()
     for old, new in (("constexpr uint32_t SMALL_SCAN_N = 1024 * SMALL_SCAN_ITEMS;", "constexpr uint32_t SMALL_SCAN_N = 0;"),
                      ("constexpr uint32_t LB_SCAN_ITEMS = 32;", "constexpr uint32_t LB_SCAN_ITEMS = 4;"),
-                     ("constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;", "constexpr uint32_t SORT_SMALL_USE = 1024;")):
+                     ("constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;", "constexpr uint32_t SORT_SMALL_USE = 1024;"),
+                     ("constexpr uint32_t SE_ROWS = 32, SE_ROWS_SHORT = 8;",
+                      "constexpr uint32_t SE_ROWS = 4, SE_ROWS_SHORT = 2;"),
+                     ("constexpr uint32_t SE_LONG = 1u << 23;", "constexpr uint32_t SE_LONG = 1500;")):
         assert kh.count(old) == 1, old
         kh = kh.replace(old, new)
     open(os.path.join(OUT, "pxd_kernels.h"), "w").write(kh)
