@@ -174,7 +174,9 @@ int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
  * clamped slots in the walk (variant 7 for the cut contribution); 11: the
  * window kernel with 2 lanes over node-index-range chunks (windows that
  * cover sparse lists; it needs the node list in index order, so a serving
- * engine, whose admitted graphs are appended, runs variant 7 for it); -1 (default): online
+ * engine, whose admitted graphs are appended, runs variant 7 for it); 12:
+ * variant 11 over half-size ranges with half the window (two workgroups a
+ * CU; a serving engine runs variant 7 for it); -1 (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
  * (b >= 16) every candidate is launched twice on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive

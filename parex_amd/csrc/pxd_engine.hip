@@ -221,7 +221,10 @@ struct pxd_engine {
     // per row (4096-entry chunks), 9 = 2 lanes per row, 1024-entry chunks and
     // a quarter of the window, 10 = variant 7 with clamped slots (walk only;
     // the cut contribution runs variant 7 for it), 11 = 2 lanes per row over
-    // node-range chunks of RANGE_R indices.  walk_force >= 0 pins one variant for both passes; -1 = online
+    // node-range chunks of RANGE_R indices, 12 = the same over chunks of
+    // RANGE_R / 2 indices with half the window (two workgroups a CU, so one
+    // stages its window while the other computes).  walk_force >= 0 pins one
+    // variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
     // every candidate is launched twice on the current state (both passes are
     // pure functions of it, all variants give the same bits) and timed, and
@@ -229,10 +232,12 @@ struct pxd_engine {
     // without tuning.  The tables survive pxd_reset, so repeated
     // decompositions of the same input reuse them.
     static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_WIN2C = 10,
-                         WV_WIN2R = 11, WV_COUNT = 12;
-    static constexpr uint32_t RANGE_R = 8192;        // node-range chunks of variant 11
-    uint32_t* d_rmap = nullptr;                      // k_range_offsets of the current list
-    bool rmap_valid = false;
+                         WV_WIN2R = 11, WV_WIN2RH = 12, WV_COUNT = 13;
+    static constexpr uint32_t RANGE_R = 8192;        // node-range chunks of variant 11 (12: half)
+    uint32_t* d_rmap = nullptr;                      // k_range_offsets of the current list, RANGE_R
+    uint32_t* d_rmap_h = nullptr;                    // the same for RANGE_R / 2
+    bool rmap_valid = false;                         // both maps belong to the current list
+    bool rmap_h_built = false;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     bool small_kernel = true;          // one-workgroup runs for small single graphs
@@ -366,6 +371,7 @@ struct pxd_engine {
         CHK(alloc(&d_ediff, N));
         CHK(alloc(&d_ped, N));
         CHK(alloc(&d_rmap, (size_t)N / RANGE_R + 2));
+        CHK(alloc(&d_rmap_h, (size_t)N / (RANGE_R / 2) + 2));
         CHK(alloc(&d_pvol, N));
         CHK(alloc(&d_root, N));
         CHK(alloc(&d_part, N));
@@ -751,17 +757,28 @@ struct pxd_engine {
     void launch_walk_win() {
         const uint32_t nch = RANGE ? cdiv(N, CHUNK) : cdiv(nalist, CHUNK);
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
-        if (RANGE) range_map();
+        const uint32_t* rm = RANGE ? range_map(CHUNK) : nullptr;
         k_walk_win<GSZ, U, WIN, CHUNK, CLAMP, RANGE><<<grid, WBS, 0, st>>>(
             nalist, d_alist[acur], d_row, d_dint, d_nbr, d_y[cur], d_x[cur], d_x[cur ^ 1], d_y[cur ^ 1], d_xkey,
-            heavy_thr, N, d_cnt, d_rmap);
+            heavy_thr, N, d_cnt, rm);
     }
-    // list entries of the node-range chunks (valid until the list changes)
-    void range_map() {
-        if (rmap_valid) return;
-        const uint32_t nch = cdiv(N, RANGE_R);
-        k_range_offsets<<<grid_for((uint64_t)nch + 1, BS), BS, 0, st>>>(nch, RANGE_R, d_alist[acur], nalist, d_rmap);
-        rmap_valid = true;
+    // list entries of the node-range chunks of R indices, R = RANGE_R or
+    // RANGE_R / 2 (valid until the list changes)
+    const uint32_t* range_map(uint32_t R) {
+        if (!rmap_valid) {
+            rmap_valid = true;
+            rmap_h_built = false;
+            const uint32_t nch = cdiv(N, RANGE_R);
+            k_range_offsets<<<grid_for((uint64_t)nch + 1, BS), BS, 0, st>>>(nch, RANGE_R, d_alist[acur], nalist, d_rmap);
+        }
+        if (R == RANGE_R) return d_rmap;
+        if (!rmap_h_built) {
+            rmap_h_built = true;
+            const uint32_t nch = cdiv(N, RANGE_R / 2);
+            k_range_offsets<<<grid_for((uint64_t)nch + 1, BS), BS, 0, st>>>(nch, RANGE_R / 2, d_alist[acur], nalist,
+                                                                             d_rmap_h);
+        }
+        return d_rmap_h;
     }
     void launch_walk_variant(int v) {
         switch (v) {
@@ -778,6 +795,10 @@ struct pxd_engine {
             case WV_WIN2R:
                 if (serving) launch_walk_win<2, 8, 20480, 4096>();   // list not in index order
                 else launch_walk_win<2, 8, 20480, RANGE_R, false, true>();
+                break;
+            case WV_WIN2RH:
+                if (serving) launch_walk_win<2, 8, 20480, 4096>();
+                else launch_walk_win<2, 8, 10240, RANGE_R / 2, false, true>();
                 break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
@@ -816,7 +837,10 @@ struct pxd_engine {
         if (pass == 0 && gs <= 8 && nalist >= (1u << 22)) cand.push_back(WV_WIN2C);
         // node-range chunks: windows that cover sparse lists too (they need the
         // list in index order, which a serving engine's appended graphs break)
-        if (gs <= 8 && !serving) cand.push_back(WV_WIN2R);
+        if (gs <= 8 && !serving) {
+            cand.push_back(WV_WIN2R);
+            cand.push_back(WV_WIN2RH);
+        }
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -847,9 +871,9 @@ struct pxd_engine {
     void launch_contrib_win() {
         const uint32_t nch = RANGE ? cdiv(N, CHUNK) : cdiv(nalist, CHUNK);
         const uint32_t grid = (std::min<uint32_t>(std::max<uint32_t>(nch, 1u), 4096u) + NXCD - 1) / NXCD * NXCD;
-        if (RANGE) range_map();
+        const uint32_t* rm = RANGE ? range_map(CHUNK) : nullptr;
         k_contrib_win<GSZ, U, WIN, CHUNK, RANGE><<<grid, WBS, 0, st>>>(nalist, d_alist[acur], d_row, d_dint, d_nbr,
-                                                                      d_xkey, d_ediff, heavy_thr, N, d_rmap);
+                                                                      d_xkey, d_ediff, heavy_thr, N, rm);
     }
     void launch_contrib_variant(int v) {
         switch (v) {
@@ -866,6 +890,10 @@ struct pxd_engine {
             case WV_WIN2R:
                 if (serving) launch_contrib_win<2, 8, 40960, 4096>();   // list not in index order
                 else launch_contrib_win<2, 8, 40960, RANGE_R, true>();
+                break;
+            case WV_WIN2RH:
+                if (serving) launch_contrib_win<2, 8, 40960, 4096>();
+                else launch_contrib_win<2, 8, 20480, RANGE_R / 2, true>();
                 break;
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }

@@ -1484,7 +1484,21 @@ __global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ p
     const uint32_t t = tidx[blockIdx.x];
     const uint32_t b = L.start + t * TILE;
     const uint32_t e = min(L.start + L.len, b + (uint32_t)TILE);
-    for (uint32_t p = b + threadIdx.x; p < e; p += BS) atomicAdd(&h[(key[p] >> shift) & 255u], 1u);
+    // a wave whose lanes share one digit adds its count with one atomic: the
+    // regroup's keys come in runs (a cluster's positions share its minimum),
+    // and 64 lanes on one LDS counter would serialise
+    for (uint32_t p0 = b; p0 < e; p0 += BS) {
+        const uint32_t p = p0 + threadIdx.x;
+        const bool ok = p < e;
+        const uint32_t d = ok ? (key[p] >> shift) & 255u : 256u;
+        const uint32_t d0 = __shfl(d, 0);
+        const uint64_t same = __ballot(ok && d == d0);
+        if (same == __ballot(ok)) {
+            if (lane_id() == 0 && d0 < 256u) atomicAdd(&h[d0], (uint32_t)__popcll(same));
+        } else if (ok) {
+            atomicAdd(&h[d], 1u);
+        }
+    }
     __syncthreads();
     hist[L.hbase + threadIdx.x * L.ntiles + t] = h[threadIdx.x];
 }
