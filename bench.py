@@ -484,7 +484,8 @@ def main():
                 # online tuner: 0..5 = 2..64 lanes per row, 6/7/8 = LDS window
                 # with 1/2/4 lanes per row, 9 = LDS window, 2 lanes, 1024-entry
                 # chunks, 10 = 7 with clamped walk slots, 11 / 12 = LDS window
-                # over node-range chunks of 8192 / 4096 indices
+                # over node-range chunks of 8192 / 4096 indices, 13 = half
+                # window over 2048-entry chunks
                 "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
                 "tunings": tunings}
     lms, lby = b["walk_log"]

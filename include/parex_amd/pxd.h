@@ -176,7 +176,9 @@ int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
  * cover sparse lists; it needs the node list in index order, so a serving
  * engine, whose admitted graphs are appended, runs variant 7 for it); 12:
  * variant 11 over half-size ranges with half the window (two workgroups a
- * CU; a serving engine runs variant 7 for it); -1 (default): online
+ * CU; a serving engine runs variant 7 for it); 13: the window kernel with 2
+ * lanes, 2048-entry chunks and half the window (two workgroups a CU); -1
+ * (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
  * (b >= 16) every candidate is launched twice on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive

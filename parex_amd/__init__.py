@@ -608,7 +608,8 @@ class DeviceManager:
         lanes per row; 6, 7, 8: LDS window with 1, 2, 4 lanes per row; 9: LDS
         window, 2 lanes, 1024-entry chunks; 10: 7 with clamped walk slots; 11:
         LDS window over node-range chunks of 8192 indices; 12: the same over
-        4096-index chunks with half the window), -1 =
+        4096-index chunks with half the window; 13: LDS window, 2 lanes,
+        2048-entry chunks, half the window), -1 =
         online tuning (default), -2 = the degree rule.  Every variant computes
         the same bits."""
         self._check(self._lib.pxd_set_walk_variant(self._h, int(variant)))

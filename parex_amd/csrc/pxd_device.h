@@ -157,6 +157,37 @@ __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1);
 // The value stays in its VGPRs as computed: the optimizer cannot fold the
 // load that produced it into a later select (no instruction is emitted).
 __device__ __forceinline__ void opaque_u64(uint64_t& v) { __asm__ volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque_u32(uint32_t& v) { __asm__ volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(uint64_t& v) { opaque_u64(v); }
+__device__ __forceinline__ void opaque(uint32_t& v) { opaque_u32(v); }
+
+// U gathers g[idx[k]] through an LDS window sw[0, win) that holds g[wb, wb +
+// win) when `use`.  Every slot reads its LDS word first (word 0 when the
+// index misses the window); the results are pinned, then the valid misses
+// load from global memory.  A miss's global load then writes a register no
+// LDS read is still writing, so the misses of all U slots are issued back to
+// back and waited for once (a per-slot choice between the two loads makes the
+// compiler wait out each miss before the next slot's LDS read).  ZERO: an
+// invalid slot gives 0, otherwise whatever its LDS read returned.
+template <int U, bool ZERO, class T>
+__device__ __forceinline__ void win_gather(T (&val)[U], const uint32_t (&idx)[U], const bool (&valid)[U],
+                                           const T* sw, const T* __restrict__ g, uint32_t wb, uint32_t win,
+                                           bool use) {
+    bool miss[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const uint32_t off = idx[k] - wb;
+        const bool inw = use && off < win;
+        const T t = sw[inw ? off : 0u];
+        val[k] = (ZERO && !valid[k]) ? T(0) : t;
+        miss[k] = valid[k] && !inw;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) opaque(val[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+        if (miss[k]) val[k] = g[idx[k]];
+}
 
 // XCD-aware split of a grid-stride loop over n items.  Workgroups are
 // dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8, each XCD

@@ -438,15 +438,14 @@ __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t
                         const uint32_t jj = j + q * GS;
                         v[q] = jj < rd ? nbr[rs + jj] : 0u;
                     }
+                    bool valid[U];
 #pragma unroll
-                    for (int q = 0; q < U; ++q) {
-                        const uint32_t off = v[q] - wb;
-                        k[q] = j + q * GS < rd ? ((use && off < (uint32_t)WIN) ? sk[off] : xkey[v[q]]) : 0u;
-                    }
+                    for (int q = 0; q < U; ++q) valid[q] = j + q * GS < rd;
+                    win_gather<U, false>(k, v, valid, sk, xkey, wb, (uint32_t)WIN, use);
 #pragma unroll
                     for (int q = 0; q < U; ++q) {
                         const bool before = (k[q] != rk) ? (k[q] < rk) : (v[q] < ru);
-                        if (j + q * GS < rd) cnt += before ? -1 : 1;
+                        if (valid[q]) cnt += before ? -1 : 1;
                     }
                 }
                 cnt = group_sum<GS>(cnt);
@@ -521,8 +520,11 @@ __global__ void k_range_offsets(uint32_t nch, uint32_t R, const uint32_t* __rest
 // entries; coff[c] (k_range_offsets) is the first list entry of chunk c.  The
 // window always covers its chunk, however sparse the list (a serving engine's
 // node space, late iterations); chunks without active nodes are skipped.
+//
+// A window of at most 80 KB asks for 8 waves per SIMD (at most 64 VGPRs), so
+// that two workgroups share a CU: one stages its window while the other sums.
 template <int GS, int U, int WIN, int CH, bool CLAMP = false, bool RANGE = false>
-__global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* __restrict__ alist,
+__global__ void __launch_bounds__(WBS, WIN * 8 <= 81920 ? 8 : 4) k_walk_win(uint32_t na, const uint32_t* __restrict__ alist,
                                                   const uint32_t* __restrict__ row,
                                                   const uint32_t* __restrict__ dint,
                                                   const uint32_t* __restrict__ nbr,
@@ -604,25 +606,12 @@ __global__ void __launch_bounds__(WBS) k_walk_win(uint32_t na, const uint32_t* _
                             q[k] = jj < rd ? nbr[rs + jj] : 0u;
                         }
                     }
-                    // every slot reads its window word from LDS; window misses
-                    // then load from global memory.  The barrier keeps the two
-                    // loads apart (the optimizer would otherwise merge them into
-                    // one flat load, which goes through the vector memory path)
-                    bool inw[U];
+                    // window words from LDS, misses from global memory, slots
+                    // past the row's end 0
+                    bool valid[U];
 #pragma unroll
-                    for (int k = 0; k < U; ++k) {
-                        const uint32_t off = q[k] - wb;
-                        inw[k] = use && off < (uint32_t)WIN;
-                        yv[k] = sy[inw[k] ? off : 0u];
-                    }
-#pragma unroll
-                    for (int k = 0; k < U; ++k) opaque_u64(yv[k]);
-#pragma unroll
-                    for (int k = 0; k < U; ++k) {
-                        const uint32_t jj = j + k * GS;
-                        if (!inw[k] && (CLAMP || jj < rd)) yv[k] = y[q[k]];
-                        if (jj >= rd) yv[k] = 0ull;
-                    }
+                    for (int k = 0; k < U; ++k) valid[k] = j + k * GS < rd;
+                    win_gather<U, true>(yv, q, valid, sy, y, wb, (uint32_t)WIN, use);
 #pragma unroll
                     for (int k = 0; k < U; ++k) fix_add_packed(acc, yv[k]);
                 }
@@ -1874,20 +1863,44 @@ __global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32
             }
         }
         __syncthreads();
-        for (uint32_t i = c0 + threadIdx.x; i < c1; i += WBS) {
-            const uint32_t u = list[i];
-            if (r >= dint[u]) continue;
-            const uint32_t v = nbr[row[u] + r];
-            const uint32_t ou = u - wb, ov = v - wb;
-            const bool iu = use && ou < (uint32_t)WIN, iv = use && ov < (uint32_t)WIN;
-            const uint32_t pu = iu ? sp_[ou] : part[u];
-            const uint32_t pv = iv ? sp_[ov] : part[v];
-            if (pv != pu) continue;
-            const uint32_t a = iu ? sr_[ou] : root[u];
-            const uint32_t b = iv ? sr_[ov] : root[v];
-            if (a == b) continue;
-            const uint32_t hi = max(a, b), lo = min(a, b);
-            if (root[hi] > lo) atomicMin(&root[hi], lo);
+        // SI members per thread at a time, each step's loads in flight together
+        constexpr int SI = 4;
+        for (uint32_t i0 = c0 + threadIdx.x; i0 < c1; i0 += SI * WBS) {
+            uint32_t u[SI], v[SI], pu[SI], pv[SI], a[SI], b[SI];
+            bool ok[SI];
+#pragma unroll
+            for (int k = 0; k < SI; ++k) {
+                ok[k] = i0 + k * WBS < c1;
+                u[k] = list[ok[k] ? i0 + k * WBS : c0];
+            }
+            uint32_t d[SI], rw[SI];
+#pragma unroll
+            for (int k = 0; k < SI; ++k) {
+                d[k] = dint[u[k]];
+                rw[k] = row[u[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < SI; ++k) {
+                ok[k] = ok[k] && r < d[k];
+                v[k] = ok[k] ? nbr[rw[k] + r] : u[k];
+            }
+            win_gather<SI, false>(pu, u, ok, sp_, part, wb, (uint32_t)WIN, use);
+            win_gather<SI, false>(pv, v, ok, sp_, part, wb, (uint32_t)WIN, use);
+#pragma unroll
+            for (int k = 0; k < SI; ++k) ok[k] = ok[k] && pu[k] == pv[k];
+            win_gather<SI, false>(a, u, ok, sr_, (const uint32_t*)root, wb, (uint32_t)WIN, use);
+            win_gather<SI, false>(b, v, ok, sr_, (const uint32_t*)root, wb, (uint32_t)WIN, use);
+            uint32_t cur[SI], hi[SI], lo[SI];
+#pragma unroll
+            for (int k = 0; k < SI; ++k) {
+                ok[k] = ok[k] && a[k] != b[k];
+                hi[k] = max(a[k], b[k]);
+                lo[k] = min(a[k], b[k]);
+                cur[k] = ok[k] ? root[hi[k]] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < SI; ++k)
+                if (ok[k] && cur[k] > lo[k]) atomicMin(&root[hi[k]], lo[k]);
         }
         __syncthreads();
     }
@@ -2410,12 +2423,13 @@ __global__ void __launch_bounds__(WBS) k_disable_win(uint32_t nl, const uint32_t
                         const uint32_t j = j0 + q * GS + gl;
                         v[q] = j < rd ? nbr[rs + j] : 0u;
                     }
+                    bool valid[U];
+                    uint32_t rv[U];
 #pragma unroll
-                    for (int q = 0; q < U; ++q) {
-                        const uint32_t j = j0 + q * GS + gl;
-                        const uint32_t off = v[q] - wb;
-                        keep[q] = j < rd && ((use && off < (uint32_t)WIN) ? sr[off] : root[v[q]]) == rr;
-                    }
+                    for (int q = 0; q < U; ++q) valid[q] = j0 + q * GS + gl < rd;
+                    win_gather<U, false>(rv, v, valid, sr, root, wb, (uint32_t)WIN, use);
+#pragma unroll
+                    for (int q = 0; q < U; ++q) keep[q] = valid[q] && rv[q] == rr;
 #pragma unroll
                     for (int q = 0; q < U; ++q) {
                         const uint32_t j = j0 + q * GS + gl;

@@ -184,7 +184,7 @@ def test_serve_edge_lists_equal_solo_and_csr_admissions():
     srv.close()
 
 
-@pytest.mark.parametrize("variant", [1, 7, 9, 10, 11, 12])
+@pytest.mark.parametrize("variant", [1, 7, 9, 10, 11, 12, 13])
 def test_serve_pinned_kernel_variants_equal_solo(variant):
     """A serving engine's node list is a concatenation of the admitted graphs'
     lists, not one list in index order: every pinned kernel variant still
