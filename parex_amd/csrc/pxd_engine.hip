@@ -371,8 +371,9 @@ struct pxd_engine {
         CHK(alloc(&d_xkey, N));
         CHK(alloc(&d_ediff, N));
         CHK(alloc(&d_ped, N));
-        CHK(alloc(&d_rmap, (size_t)N / RANGE_R + 2));
-        CHK(alloc(&d_rmap_h, (size_t)N / (RANGE_R / 2) + 2));
+        // chunk offsets (nch + 1) and the XCD split (NXCD + 1)
+        CHK(alloc(&d_rmap, (size_t)N / RANGE_R + 3 + NXCD));
+        CHK(alloc(&d_rmap_h, (size_t)N / (RANGE_R / 2) + 3 + NXCD));
         CHK(alloc(&d_pvol, N));
         CHK(alloc(&d_root, N));
         CHK(alloc(&d_part, N));

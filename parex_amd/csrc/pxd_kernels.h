@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
         const uint32_t u = alist[i];
         const uint32_t start = row[u], d = dint[u];
         if (d > heavy) continue;      // rows above the threshold go to k_walk_heavy
-        Fix128 acc = {0ull, 0ull};
+        FixAcc fa = {0ull, 0};
         // four independent index loads, then four term gathers in flight per
         // lane; slots past the row end re-read the row's last arc (no branch,
         // same line) and add a zero term
@@ -302,12 +302,13 @@ __global__ void __launch_bounds__(BS) k_walk(uint32_t na, const uint32_t* __rest
             const uint32_t v2 = p[min(j + 2 * GS, d - 1)];
             const uint32_t v3 = p[min(j + 3 * GS, d - 1)];
             const uint64_t y0 = y[v0], y1 = y[v1], y2 = y[v2], y3 = y[v3];
-            fix_add_packed(acc, y0);
-            fix_add_packed(acc, b1 ? y1 : 0ull);
-            fix_add_packed(acc, b2 ? y2 : 0ull);
-            fix_add_packed(acc, b3 ? y3 : 0ull);
+            fixacc_add(fa, y0);
+            fixacc_add(fa, b1 ? y1 : 0ull);
+            fixacc_add(fa, b2 ? y2 : 0ull);
+            fixacc_add(fa, b3 ? y3 : 0ull);
+            fixacc_norm(fa);
         }
-        acc = group_sum_fix<GS>(acc);
+        const Fix128 acc = fixacc_fix(group_sum_fixacc<GS>(fa));
         if (lane == 0) walk_finish(u, d, acc, xc, xn, yn, xkey, ovf);
     }
     if (ovf) atomicOr(&cnt->err, ERR_FIX_RANGE);
@@ -379,7 +380,9 @@ __global__ void __launch_bounds__(WBS) k_contrib_win(uint32_t na, const uint32_t
     const uint32_t nch = RANGE ? (N + CH - 1) / CH : (na + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
-    const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
+    // range chunks: the XCD split by active entries (k_range_offsets)
+    const uint32_t cb = RANGE ? coff[nch + 1 + x] : min(nch, x * cpx);
+    const uint32_t ce = RANGE ? coff[nch + 2 + x] : min(nch, cb + cpx);
     for (uint32_t c = cb + lb; c < ce; c += nb) {
         uint32_t c0, c1, mid;
         bool use;
@@ -485,17 +488,34 @@ __global__ void k_heavy_expand(uint32_t n, const uint32_t* __restrict__ hcnt, co
 }
 
 // coff[c] = first entry of the ascending node list at or after node c*R, for
-// c = 0..nch (coff[nch] = n): the list entries of the node range chunks
+// c = 0..nch (coff[nch] = n): the list entries of the node range chunks.
+// Then the XCD split by entries: coff[nch + 1 + x] (x = 0..NXCD) = the first
+// chunk whose entries start at or after x/NXCD of the list (x = NXCD: nch),
+// so that every XCD gets about the same number of active nodes however they
+// cluster in index order.
+__device__ __forceinline__ uint32_t range_lower(uint32_t c, uint32_t nch, uint32_t R,
+                                                const uint32_t* __restrict__ alist, uint32_t n) {
+    if (c >= nch) return n;
+    const uint64_t key = (uint64_t)c * R;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t m = lo + (hi - lo) / 2;
+        if ((uint64_t)alist[m] < key) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
 __global__ void k_range_offsets(uint32_t nch, uint32_t R, const uint32_t* __restrict__ alist, uint32_t n,
                                 uint32_t* __restrict__ coff) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nch; c += gridDim.x * blockDim.x) {
-        const uint64_t key = (uint64_t)c * R;
-        uint32_t lo = 0, hi = n;
-        while (lo < hi) {
-            const uint32_t m = lo + (hi - lo) / 2;
-            if ((uint64_t)alist[m] < key) lo = m + 1; else hi = m;
+        const uint32_t lo = range_lower(c, nch, R, alist, n);
+        const uint32_t prev = c ? range_lower(c - 1, nch, R, alist, n) : 0u;
+        coff[c] = lo;
+        // chunk c is XCD x's first when target_x lies in (prev, lo] (c = 0: target_x = 0)
+        for (uint32_t x = 0; x <= NXCD; ++x) {
+            const uint32_t t = (uint32_t)((uint64_t)x * n / NXCD);
+            const bool first = x == NXCD ? c == nch : (c == 0 ? t == 0u : (prev < t && t <= lo));
+            if (first) coff[nch + 1 + x] = c;
         }
-        coff[c] = c == nch ? n : lo;
     }
 }
 
@@ -539,11 +559,13 @@ __global__ void __launch_bounds__(WBS, WIN * 8 <= 81920 ? 8 : 4) k_walk_win(uint
     constexpr uint32_t NW = WBS / 64;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     const uint32_t g = lane / GS, gl = lane % GS;
-    // XCD-aware chunk split (see xcd_span): XCD x takes the x-th eighth of the chunks
+    // XCD-aware chunk split (see xcd_span): XCD x takes the x-th eighth of the
+    // chunks; range chunks are split by active entries (k_range_offsets)
     const uint32_t nch = RANGE ? (N + CH - 1) / CH : (na + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
-    const uint32_t cb = min(nch, x * cpx), ce = min(nch, cb + cpx);
+    const uint32_t cb = RANGE ? coff[nch + 1 + x] : min(nch, x * cpx);
+    const uint32_t ce = RANGE ? coff[nch + 2 + x] : min(nch, cb + cpx);
     uint32_t ovf = 0;
     for (uint32_t c = cb + lb; c < ce; c += nb) {
         uint32_t c0, c1, mid;
@@ -587,12 +609,12 @@ __global__ void __launch_bounds__(WBS, WIN * 8 <= 81920 ? 8 : 4) k_walk_win(uint
             const bool mine = ok && dd <= heavy;
             const uint32_t d = mine ? dd : 0u;
             const double xo = xc[u];
-            Fix128 res = {0ull, 0ull};
+            FixAcc res = {0ull, 0};
 #pragma unroll
             for (int r = 0; r < GS; ++r) {
                 const uint32_t s = r * NPR + g;
                 const uint32_t rs = __shfl(st, s), rd = __shfl(d, s);
-                Fix128 acc = {0ull, 0ull};
+                FixAcc acc = {0ull, 0};
                 for (uint32_t j = gl; j < rd; j += U * GS) {
                     uint32_t q[U];
                     uint64_t yv[U];
@@ -613,16 +635,18 @@ __global__ void __launch_bounds__(WBS, WIN * 8 <= 81920 ? 8 : 4) k_walk_win(uint
                     for (int k = 0; k < U; ++k) valid[k] = j + k * GS < rd;
                     win_gather<U, true>(yv, q, valid, sy, y, wb, (uint32_t)WIN, use);
 #pragma unroll
-                    for (int k = 0; k < U; ++k) fix_add_packed(acc, yv[k]);
+                    for (int k = 0; k < U; ++k) fixacc_add(acc, yv[k]);
+                    fixacc_norm(acc);
                 }
-                acc = group_sum_fix<GS>(acc);
+                acc = group_sum_fixacc<GS>(acc);
                 const uint32_t src = (lane % NPR) * GS;
-                const uint64_t lo = __shfl(acc.lo, src), hi = __shfl(acc.hi, src);
-                if (lane / NPR == (uint32_t)r) res = Fix128{lo, hi};
+                const uint64_t lo = __shfl(acc.lo, src);
+                const int64_t hi = __shfl(acc.hi, src);
+                if (lane / NPR == (uint32_t)r) res = FixAcc{lo, hi};
             }
             if (mine) {
                 if (d > (1u << 24)) ovf = 1;
-                const double sum = fix_to_f64(res);
+                const double sum = fix_to_f64(fixacc_fix(res));
                 const double v = sum * (1.0 - RW_STAY) + xo * RW_STAY;
                 xn[u] = v;
                 yn[u] = fix_pack(d ? v / (double)d : 0.0, ovf);
@@ -674,10 +698,10 @@ __global__ void __launch_bounds__(BS) k_walk_heavy(const uint32_t* __restrict__ 
         uint64_t yv[4];
 #pragma unroll
         for (uint32_t q = 0; q < 4; ++q) yv[q] = y[v[q]];
-        Fix128 acc = {0ull, 0ull};
+        FixAcc fa = {0ull, 0};
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) fix_add_packed(acc, ok[q] ? yv[q] : 0ull);
-        acc = block_sum_fix(acc, lds);
+        for (uint32_t q = 0; q < 4; ++q) fixacc_add(fa, ok[q] ? yv[q] : 0ull);
+        Fix128 acc = block_sum_fix(fixacc_fix(fa), lds);
         if (threadIdx.x == 0) part[b] = acc;
         __syncthreads();              // lds reused by the next chunk
     }

@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
                 const uint32_t u = alist[e];
                 const uint32_t st = a.row[u], d = a.dint[u];
                 if (d > SM_LONG) continue;
-                Fix128 acc = {0ull, 0ull};
+                FixAcc fa = {0ull, 0};
                 for (uint32_t j = 0; j < d; j += 4) {
                     uint32_t v[4];
                     uint64_t q[4];
@@ -177,9 +177,9 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
 #pragma unroll
                     for (uint32_t z = 0; z < 4; ++z) q[z] = yc[v[z]];
 #pragma unroll
-                    for (uint32_t z = 0; z < 4; ++z) fix_add_packed(acc, j + z < d ? q[z] : 0ull);
+                    for (uint32_t z = 0; z < 4; ++z) fixacc_add(fa, j + z < d ? q[z] : 0ull);
                 }
-                walk_finish(u, d, acc, xc, xn, yn, a.xkey, ovf);
+                walk_finish(u, d, fixacc_fix(fa), xc, xn, yn, a.xkey, ovf);
             }
             // longer rows: 16 lanes each, four rows of a wave at a time
             for (uint32_t b = w * WAVE; b < nalist; b += SM_NT) {
@@ -191,9 +191,12 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
                     const uint32_t ur = __shfl(u, (int)(src & 63u)), dr = __shfl(d, (int)(src & 63u));
                     if (src < (uint32_t)WAVE) {            // uniform within the 16-lane group
                         const uint32_t st = a.row[ur], gl = l % SM_GL;
-                        Fix128 acc = {0ull, 0ull};
-                        for (uint32_t j = gl; j < dr; j += SM_GL) fix_add_packed(acc, yc[a.nbr[st + j]]);
-                        acc = group_sum_fix<SM_GL>(acc);
+                        FixAcc fa = {0ull, 0};
+                        for (uint32_t j = gl; j < dr; j += SM_GL) {
+                            fixacc_add(fa, yc[a.nbr[st + j]]);
+                            fixacc_norm(fa);
+                        }
+                        const Fix128 acc = fixacc_fix(group_sum_fixacc<SM_GL>(fa));
                         if (gl == 0) walk_finish(ur, dr, acc, xc, xn, yn, a.xkey, ovf);
                     }
                 }
