@@ -162,7 +162,9 @@ int pxd_walk_launch_log(pxd_engine* e, double* ms, double* alg_bytes, uint32_t c
  * of a single graph of at most 4096 nodes and 2^19 arcs, started right after
  * pxd_initialize or pxd_reset, runs every iteration inside one workgroup in
  * one launch, with the same results as the per-phase kernels.  enable = 1
- * (default) allows it, 0 keeps the per-phase kernels. */
+ * (default) takes it for graphs of at most 2^16 arcs (denser graphs run
+ * faster on the per-phase kernels), 2 for every graph within its capacity,
+ * 0 never. */
 int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
 
 /* Kernel choice for the walk and the cut contribution, the two per-arc

@@ -598,9 +598,10 @@ class DeviceManager:
                                                    by.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
         return ms, by
 
-    def setSmallGraphKernel(self, enable: bool):
-        """Allow (default) or forbid the one-workgroup run of a small single
-        graph (at most 4096 nodes, 2^19 arcs) in expanderDecomposition."""
+    def setSmallGraphKernel(self, enable):
+        """The one-workgroup run of a small single graph (at most 4096 nodes)
+        in expanderDecomposition: True / 1 (default) for graphs of at most
+        2^16 arcs, 2 for every graph of at most 2^19 arcs, False / 0 never."""
         self._check(self._lib.pxd_set_small_graph_kernel(self._h, int(enable)))
 
     def setWalkVariant(self, variant: int):

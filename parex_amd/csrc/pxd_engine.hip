@@ -241,7 +241,7 @@ struct pxd_engine {
     bool rmap_h_built = false;
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
-    bool small_kernel = true;          // one-workgroup runs for small single graphs
+    int small_kernel = 1;              // one-workgroup runs: 0 never, 1 up to SMALL_AUTO_ARCS, 2 always
     int tuned[2][33];
     uint64_t tuned_for[3] = {~0ull, ~0ull, ~0ull};   // (graphs, nodes, arcs) the tables were tuned on
     uint64_t walk_tunings = 0;
@@ -1588,7 +1588,8 @@ struct pxd_engine {
     // reset, with the per-arc kernels left to the engine's choice and no
     // per-launch walk timing: the whole run is one workgroup's launch.
     bool small_run_applies() const {
-        return small_kernel && !serving && G == 1 && N > 0 && N <= SMALL_MAX_N && A <= SMALL_MAX_ARCS &&
+        return small_kernel != 0 && !serving && G == 1 && N > 0 && N <= SMALL_MAX_N && A <= SMALL_MAX_ARCS &&
+               (small_kernel == 2 || A <= SMALL_AUTO_ARCS) &&
                walk_force < 0 && !walk_timing && iter1 == 0 && cur == 0 && acur == 0 && ctc == 0 &&
                na == N && nalist == N && nc == 1 && !sweep_done;
     }
@@ -2113,7 +2114,8 @@ int pxd_walk_stats(pxd_engine* e, uint64_t* launches, double* total_ms, double* 
 int pxd_set_small_graph_kernel(pxd_engine* e, int enable) {
     return guarded([&]() -> int {
         if (!e) return set_err(PXD_ERR_ARG, "null engine");
-        e->small_kernel = enable != 0;
+        if (enable < 0 || enable > 2) return set_err(PXD_ERR_ARG, "small-graph mode %d is not 0, 1 or 2", enable);
+        e->small_kernel = enable;
         return 0;
     });
 }

@@ -26,7 +26,13 @@ namespace pxd {
 constexpr int SM_NT = 1024;                          // threads of the workgroup
 constexpr int SM_ITEMS = 4;                          // items per thread in tile passes
 constexpr uint32_t SMALL_MAX_N = (uint32_t)SM_NT * SM_ITEMS;   // 4096 nodes
-constexpr uint32_t SMALL_MAX_ARCS = 1u << 19;        // beyond this the multi-kernel walk wins
+constexpr uint32_t SMALL_MAX_ARCS = 1u << 19;        // capacity of the one-workgroup run
+// The automatic choice takes the one-workgroup run up to this many arcs: it
+// costs about 20 us + 1.8 ns an arc per iteration on one CU (4096-node SBM,
+// 52,918 arcs: 116 us; a 1,714-node level of the expander hierarchy, 291,712
+// arcs: 550 us), the per-phase kernels about 170-270 us an iteration at
+// these sizes.
+constexpr uint32_t SMALL_AUTO_ARCS = 1u << 16;
 constexpr uint32_t SM_LONG = 32;                     // longer rows: 16 lanes each, else one thread
 constexpr uint32_t SM_GL = 16;                       // lanes per long row: four long rows a wave
 constexpr uint32_t SM_NW = SM_NT / WAVE;

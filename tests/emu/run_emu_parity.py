@@ -94,7 +94,7 @@ def main(lib_path, names):
             continue
         # "graph@v": the per-arc kernels pinned to variant v (pxd_set_walk_variant);
         # "graph/phases": the per-phase kernels instead of the one-workgroup run
-        # that a small single graph takes by default
+        # (forced otherwise, whatever the graph's arc count)
         name, _, mode = name.partition("/")
         gname, _, var = name.partition("@")
         g = GRAPHS[gname]()
@@ -104,8 +104,7 @@ def main(lib_path, names):
         chk(L.pxd_create(C.byref(p), C.byref(h)))
         if var:
             chk(L.pxd_set_walk_variant(h, int(var)))
-        if mode == "phases":
-            chk(L.pxd_set_small_graph_kernel(h, 0))
+        chk(L.pxd_set_small_graph_kernel(h, 0 if mode == "phases" else 2))
         r, e = np.ascontiguousarray(g.ranges, np.uint32), np.ascontiguousarray(g.edges, np.uint32)
         chk(L.pxd_initialize(h, g.n, g.m, ptr(r), ptr(e)))
         x0 = np.zeros(g.n)

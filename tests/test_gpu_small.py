@@ -1,9 +1,10 @@
 """The one-workgroup run of small single graphs (k_decompose_small) on the GPU.
 
 pxd_expander_decomposition of one graph of at most 4096 nodes and 2^19 arcs,
-straight after initialize, runs every iteration inside one workgroup.  Every
-case below is decomposed three ways from the same initial vector: that run,
-the per-phase kernels (pxd_set_small_graph_kernel(0)) and the oracle.  The
+straight after initialize, runs every iteration inside one workgroup when
+forced (pxd_set_small_graph_kernel(2); the default takes it up to 2^16 arcs).
+Every case below is decomposed three ways from the same initial vector: that
+run, the per-phase kernels (pxd_set_small_graph_kernel(0)) and the oracle.  The
 iteration count, the final partition, the labels, the internal degrees and
 the final walk values must be identical.  A capped run (max_iter) must leave
 the engine in the state the per-phase kernels leave it in: step calls after
@@ -42,7 +43,7 @@ def cases():
 
 def _run(g, small, **kw):
     dm = px.DeviceManager(px.Params(**kw))
-    dm.setSmallGraphKernel(small)
+    dm.setSmallGraphKernel(2 if small else 0)
     dm.initialize(g)
     x0 = dm.readRandomWalkValuesF64()
     it = dm.expanderDecomposition()
@@ -115,7 +116,7 @@ def test_capped_run_hands_over_the_engine_state(cap):
     out = []
     for small in (True, False):
         dm = px.DeviceManager(px.Params(max_iter=cap))
-        dm.setSmallGraphKernel(small)
+        dm.setSmallGraphKernel(2 if small else 0)
         dm.initialize(g)
         it = dm.expanderDecomposition()
         rec = [it, dm.downloadLabels(), dm.readRandomWalkValuesF64().view(np.uint64)]
@@ -133,3 +134,24 @@ def test_capped_run_hands_over_the_engine_state(cap):
     assert a[0] == b[0]
     for i, (u, v) in enumerate(zip(a[1:], b[1:])):
         assert np.array_equal(np.asarray(u), np.asarray(v)), f"cap {cap}: record {i + 1} differs"
+
+
+def test_automatic_choice_bounds_the_arcs():
+    """The default mode (1) chooses by arc count (a 4096-node SBM of 52,918
+    arcs below 2^16, a denser one of the same size above); whichever path it
+    takes, the results equal forcing the one-workgroup run (2) and the
+    per-phase kernels (0).  A mode outside 0..2 is an error."""
+    sparse = px.sbm(4096, 16, 12 / 255, 1 / 3840, 6582)
+    dense = px.sbm(4096, 8, 30 / 511, 2 / 3584, 3)
+    assert 2 * sparse.m <= 1 << 16 < 2 * dense.m
+    for g in (sparse, dense):
+        res = []
+        for mode in (1, 2, 0):
+            dm = px.DeviceManager(px.Params())
+            dm.setSmallGraphKernel(mode)
+            dm.initialize(g)
+            res.append((dm.expanderDecomposition(), dm.getFinalPartition()[0]))
+            dm.close()
+        assert all(r[0] == res[0][0] and np.array_equal(r[1], res[0][1]) for r in res)
+    with pytest.raises(px.PxdError):
+        px.DeviceManager(px.Params()).setSmallGraphKernel(3)
