@@ -194,6 +194,7 @@ struct pxd_engine {
     uint32_t maxdeg = 0, heavy_thr = 0xFFFFFFFFu, heavy_cap = 0;
     bool heavy_valid = false;
     uint32_t* d_cclist2 = nullptr;
+    uint32_t* d_ccsamp = nullptr;                    // CC sampling: arcs 1.. of every member
     unsigned long long* d_u64 = nullptr;
     Counters* d_cnt = nullptr;
     Counters hc{};
@@ -432,6 +433,7 @@ struct pxd_engine {
         CHK(alloc(&d_gclu, G));
         CHK(alloc(&d_giant, 2 * n1 + 2));
         CHK(alloc(&d_cclist2, N));
+        CHK(alloc(&d_ccsamp, (size_t)(CC_SAMPLE_ROUNDS - 1) * N));
         CHK(alloc(&d_u64, 1));
         CHK(alloc(&d_sout, 8));
         CHK(alloc(&d_cnt, 1));
@@ -1168,9 +1170,10 @@ struct pxd_engine {
                 // the entries of a 4096 chunk (C4: 382 us instead of 419 us a launch)
                 const uint32_t grid = (std::min<uint32_t>(cdiv(nl, 8192), 4096u) + NXCD - 1) / NXCD * NXCD;
                 k_cc_sample_win<20480, 8192><<<grid, WBS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part,
-                                                                   d_root, r, N);
+                                                                   d_root, r, N, d_ccsamp);
             } else {
-                k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r);
+                k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r,
+                                                             d_ccsamp);
             }
             k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
         }

@@ -1826,7 +1826,25 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
 // between rounds makes most endpoints of the next round share a pointer, so
 // later rounds issue few atomics.  Plain reads may be stale; a stale pointer
 // is still a smaller node of the same component, so it only weakens a link.
+// Round 0 reads the first CC_SAMPLE_ROUNDS arcs of every member (one line of
+// its row) and keeps arcs 1.. in samp[(r - 1) * nl + i] (INVALID past the
+// row's end), so that later rounds read them coalesced instead of a line of
+// every row again.
 constexpr uint32_t CC_SAMPLE_ROUNDS = 3;
+
+// the sampled neighbour of list entry i in round r (INVALID: none)
+__device__ __forceinline__ uint32_t cc_sample_arc(uint32_t r, uint32_t i, uint32_t u, uint32_t nl,
+                                                  const uint32_t* __restrict__ row,
+                                                  const uint32_t* __restrict__ dint,
+                                                  const uint32_t* __restrict__ nbr, uint32_t* __restrict__ samp) {
+    if (r != 0u) return samp[(r - 1u) * nl + i];
+    const uint32_t d = dint[u], st = row[u];
+    uint32_t v0 = INVALID;
+    if (d > 0u) v0 = nbr[st];
+#pragma unroll
+    for (uint32_t k = 1; k < CC_SAMPLE_ROUNDS; ++k) samp[(k - 1u) * nl + i] = d > k ? nbr[st + k] : INVALID;
+    return v0;
+}
 
 __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ row,
@@ -1834,13 +1852,12 @@ __global__ void __launch_bounds__(BS) k_cc_sample(uint32_t nl, const uint32_t* _
                                                   const uint32_t* __restrict__ nbr,
                                                   const uint32_t* __restrict__ part,
                                                   uint32_t* __restrict__ root, uint32_t r,
-                                                  const uint32_t* pn = nullptr) {
-    nl = dev_count(nl, pn);
+                                                  uint32_t* __restrict__ samp) {
     const XcdSpan sp = xcd_span(nl, BS, threadIdx.x);
     for (uint32_t i = sp.first; i < sp.end; i += sp.stride) {
         const uint32_t u = list[i];
-        if (r >= dint[u]) continue;
-        const uint32_t v = nbr[row[u] + r];
+        const uint32_t v = cc_sample_arc(r, i, u, nl, row, dint, nbr, samp);
+        if (v == INVALID) continue;
         if (part[v] != part[u]) continue;
         const uint32_t a = root[u], b = root[v];
         if (a == b) continue;
@@ -1862,9 +1879,8 @@ __global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32
                                                        const uint32_t* __restrict__ nbr,
                                                        const uint32_t* __restrict__ part,
                                                        uint32_t* __restrict__ root, uint32_t r, uint32_t N,
-                                                       const uint32_t* pn = nullptr) {
+                                                       uint32_t* __restrict__ samp) {
     __shared__ uint32_t sp_[WIN], sr_[WIN];
-    nl = dev_count(nl, pn);
     const uint32_t nch = (nl + CH - 1) / CH;
     const uint32_t x = blockIdx.x % NXCD, lb = blockIdx.x / NXCD, nb = gridDim.x / NXCD;
     const uint32_t cpx = nch / NXCD + (nch % NXCD != 0);
@@ -1897,16 +1913,11 @@ __global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32
                 ok[k] = i0 + k * WBS < c1;
                 u[k] = list[ok[k] ? i0 + k * WBS : c0];
             }
-            uint32_t d[SI], rw[SI];
 #pragma unroll
             for (int k = 0; k < SI; ++k) {
-                d[k] = dint[u[k]];
-                rw[k] = row[u[k]];
-            }
-#pragma unroll
-            for (int k = 0; k < SI; ++k) {
-                ok[k] = ok[k] && r < d[k];
-                v[k] = ok[k] ? nbr[rw[k] + r] : u[k];
+                v[k] = ok[k] ? cc_sample_arc(r, i0 + k * WBS, u[k], nl, row, dint, nbr, samp) : INVALID;
+                ok[k] = v[k] != INVALID;
+                if (!ok[k]) v[k] = u[k];
             }
             win_gather<SI, false>(pu, u, ok, sp_, part, wb, (uint32_t)WIN, use);
             win_gather<SI, false>(pv, v, ok, sp_, part, wb, (uint32_t)WIN, use);
