@@ -1064,7 +1064,7 @@ struct pxd_engine {
             // the CC list in node order (the active list holds the positions' nodes);
             // its length is the member count the sweep returned
             HIPC(hipGetLastError());
-            CHK(select(nalist, SelMembers{d_alist[acur], d_stamp, iter1, d_cclist}));
+            CHK(select(nalist, SelMembers{d_alist[acur], d_stamp, iter1, d_cclist, d_root}));
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);

@@ -1055,13 +1055,21 @@ __global__ void __launch_bounds__(BS) k_scan_emit(uint32_t n, F f, unsigned long
 
 // k_scan_emit selections of the iteration:
 // the CC members in node order: active nodes stamped by this cut
+// (and every member's pointer set to itself, in node order)
 struct SelMembers {
     const uint32_t* alist;
     const uint32_t* stamp;
     uint32_t iter;
     uint32_t* out;
+    uint32_t* root;
     __device__ bool flag(uint32_t i) const { return stamp[alist[i]] == iter; }
-    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = alist[i]; }
+    __device__ void emit(uint32_t i, uint32_t r, bool f) const {
+        if (f) {
+            const uint32_t u = alist[i];
+            out[r] = u;
+            root[u] = u;
+        }
+    }
 };
 // the members outside their part's giant tree
 struct SelOffGiant {
@@ -1792,8 +1800,9 @@ __global__ void k_decide(uint32_t nc, const unsigned long long* __restrict__ cbe
 }
 
 // cuda/devPartition.h:545-622: positions beyond the cut offset form the second
-// part.  Members of CC clusters get a part id, a fresh root and this
-// iteration's stamp (the CC list is built from the stamps in node order).
+// part.  Members of CC clusters get a part id and this iteration's stamp (the
+// CC list is built from the stamps in node order, and SelMembers then gives
+// every member a fresh root, a write in node order instead of a scattered one).
 __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __restrict__ pnix,
                                              const uint32_t* __restrict__ pclu,
                                              const uint32_t* __restrict__ cs, const uint32_t* __restrict__ cflag,
@@ -1807,8 +1816,7 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
             const uint32_t u = pnix[k];
             const uint32_t side = ((f & CF_CUT) && (k - cs[c] + 1u > coff[c])) ? 1u : 0u;
             part[u] = (c << 1) | side;
-            root[u] = u;
-            stamp[u] = iter;
+            stamp[u] = iter;          // SelMembers sets root[u] = u, in node order
         }
     }
 }
