@@ -919,7 +919,7 @@ struct pxd_engine {
                                                                     d_root, d_x[cur], d_y[cur], heavy_thr, N,
                                                                     &d_cnt->err);
         else if (lanes == 2)
-            k_disable_win<2, 4, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
+            k_disable_win<2, 8, 40960, CHUNK><<<grid, WBS, 0, st>>>(nl, d_cclist, d_alive, d_row, d_dint, d_nbr,
                                                                     d_root, d_x[cur], d_y[cur], heavy_thr, N,
                                                                     &d_cnt->err);
         else
@@ -1067,6 +1067,7 @@ struct pxd_engine {
             CHK(select(nalist, SelMembers{d_alist[acur], d_stamp, iter1, d_cclist, d_root}));
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
+            k_cc_flatten<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
             k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
             k_one_segment<<<1, 64, 0, st>>>(d_oneseg, na, nullptr);
             HIPC(hipGetLastError());
@@ -1210,7 +1211,7 @@ struct pxd_engine {
             // every changed round lowers a pointer, so this bound is never reached
             if (r >= 100000) return set_err(PXD_ERR_STATE, "connected components did not converge (%u members)", nl);
         }
-        return 0;                     // k_rid flattens every member while it reads the roots
+        return 0;                     // k_cc_flatten then sets every member's pointer to its root
     }
 
     // ---------------------------------------------------------------- serving

@@ -2096,21 +2096,29 @@ __global__ void __launch_bounds__(BS) k_cc_jump(uint32_t nl, const uint32_t* __r
     }
 }
 
-// regroup key: component root for CC members, the cluster's minimum node otherwise
-// (CC members also store their root as their pointer: the trees end flat)
+// After the components have converged: every member's pointer set to its
+// root, over the member list in node order (the first read of every chase and
+// the write stream; the next hops mostly hit one cached root).  No root moves
+// any more, so cached loads see pointers on the same path to the same root; a
+// write sets the thread's own slot (agent scope), and a reader sees the old or
+// the new pointer.  The trees end flat, for k_rid and arc disabling.
+__global__ void k_cc_flatten(uint32_t nl, const uint32_t* __restrict__ list, uint32_t* __restrict__ root) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+        const uint32_t u = list[i];
+        const uint32_t p = root[u];
+        const uint32_t r = p == u ? u : cc_find_stable(root, p);
+        if (r != p) st_dev(&root[u], r);
+    }
+}
+
+// regroup key: component root for CC members (their pointers, flat after
+// k_cc_flatten), the cluster's minimum node otherwise
 __global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
                       const uint32_t* __restrict__ cflag, const uint32_t* __restrict__ cmin,
-                      uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
+                      const uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
         const uint32_t c = pclu[k];
-        if (cflag[c] & CF_CC) {
-            const uint32_t u = pnix[k];
-            const uint32_t r = cc_find_stable(root, u);
-            st_dev(&root[u], r);      // own slot; readers see the old or the new pointer, both on u's path
-            key[k] = r;
-        } else {
-            key[k] = cmin[c];
-        }
+        key[k] = (cflag[c] & CF_CC) ? root[pnix[k]] : cmin[c];
     }
 }
 
