@@ -182,7 +182,7 @@ int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
  * lanes, 2048-entry chunks and half the window (two workgroups a CU); -1
  * (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
- * (b >= 16) every candidate is launched twice on the current state and timed,
+ * (b >= 16) every candidate is launched three times on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive
  * pxd_reset); -2: the degree rule without tuning.  Every variant computes the
  * same bits.  pxd_walk_tuning copies the tables (entries 0..32: walk per

@@ -228,7 +228,7 @@ struct pxd_engine {
     // the window over 2048-entry chunks (two workgroups a CU).  walk_force >= 0 pins one
     // variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
-    // every candidate is launched twice on the current state (both passes are
+    // every candidate is launched three times on the current state (both passes are
     // pure functions of it, all variants give the same bits) and timed, and
     // the fastest is kept for that pass and length class; -2 = the degree rule
     // without tuning.  The tables survive pxd_reset, so repeated
@@ -851,7 +851,7 @@ struct pxd_engine {
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
-            for (int rep = 0; rep < 2; ++rep) {          // the faster of two launches
+            for (int rep = 0; rep < 3; ++rep) {          // the fastest of three launches
                 if (hipEventRecord(tev0, st) != hipSuccess) return def;
                 launch(v);
                 if (hipGetLastError() != hipSuccess) break;   // a candidate that cannot launch drops out

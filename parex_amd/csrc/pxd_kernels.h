@@ -1838,7 +1838,7 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
 // its row) and keeps arcs 1.. in samp[(r - 1) * nl + i] (INVALID past the
 // row's end), so that later rounds read them coalesced instead of a line of
 // every row again.
-constexpr uint32_t CC_SAMPLE_ROUNDS = 3;
+constexpr uint32_t CC_SAMPLE_ROUNDS = 4;
 
 // the sampled neighbour of list entry i in round r (INVALID: none)
 __device__ __forceinline__ uint32_t cc_sample_arc(uint32_t r, uint32_t i, uint32_t u, uint32_t nl,
