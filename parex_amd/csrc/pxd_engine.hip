@@ -1176,7 +1176,9 @@ struct pxd_engine {
                 k_cc_sample<<<grid_xcd(nl, BS), BS, 0, st>>>(nl, d_cclist, d_row, d_dint, d_nbr, d_part, d_root, r,
                                                              d_ccsamp);
             }
-            k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
+            // no jump after the last round: the giant roots and the off-giant
+            // selection follow pointers themselves
+            if (r + 1 < CC_SAMPLE_ROUNDS) k_cc_jump<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
         }
         k_cc_giant<<<grid_for(nc, BS), BS, 0, st>>>(nc, T.cs, T.cflag, d_coff, d_pnix[0], d_root, d_giant, d_cnt);
         HIPC(hipGetLastError());
