@@ -485,7 +485,8 @@ def main():
                 # with 1/2/4 lanes per row, 9 = LDS window, 2 lanes, 1024-entry
                 # chunks, 10 = 7 with clamped walk slots, 11 / 12 = LDS window
                 # over node-range chunks of 8192 / 4096 indices, 13 = half
-                # window over 2048-entry chunks
+                # window over 2048-entry chunks, 14 / 15 = 11 / 12 with one lane
+                # per row
                 "walk_variant_by_log2_nodes": tuning, "contrib_variant_by_log2_nodes": ctuning,
                 "tunings": tunings}
     lms, lby = b["walk_log"]

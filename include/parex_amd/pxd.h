@@ -179,8 +179,9 @@ int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
  * engine, whose admitted graphs are appended, runs variant 7 for it); 12:
  * variant 11 over half-size ranges with half the window (two workgroups a
  * CU; a serving engine runs variant 7 for it); 13: the window kernel with 2
- * lanes, 2048-entry chunks and half the window (two workgroups a CU); -1
- * (default): online
+ * lanes, 2048-entry chunks and half the window (two workgroups a CU); 14,
+ * 15: variants 11, 12 with one lane per row (a serving engine runs variant 6
+ * for them); -1 (default): online
  * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
  * (b >= 16) every candidate is launched three times on the current state and timed,
  * the fastest is kept for that pass and length class (the tables survive

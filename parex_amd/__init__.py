@@ -610,7 +610,8 @@ class DeviceManager:
         window, 2 lanes, 1024-entry chunks; 10: 7 with clamped walk slots; 11:
         LDS window over node-range chunks of 8192 indices; 12: the same over
         4096-index chunks with half the window; 13: LDS window, 2 lanes,
-        2048-entry chunks, half the window), -1 =
+        2048-entry chunks, half the window; 14 / 15: 11 / 12 with one lane per
+        row), -1 =
         online tuning (default), -2 = the degree rule.  Every variant computes
         the same bits."""
         self._check(self._lib.pxd_set_walk_variant(self._h, int(variant)))

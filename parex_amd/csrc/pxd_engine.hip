@@ -225,7 +225,9 @@ struct pxd_engine {
     // node-range chunks of RANGE_R indices, 12 = the same over chunks of
     // RANGE_R / 2 indices with half the window (two workgroups a CU, so one
     // stages its window while the other computes), 13 = variant 7 with half
-    // the window over 2048-entry chunks (two workgroups a CU).  walk_force >= 0 pins one
+    // the window over 2048-entry chunks (two workgroups a CU), 14 / 15 = variants
+    // 11 / 12 with one lane per row (short rows: one round of 64 rows a wave).
+    // walk_force >= 0 pins one
     // variant for both passes; -1 = online
     // tuning: the first time the node list has a length in [2^b, 2^(b+1)),
     // every candidate is launched three times on the current state (both passes are
@@ -234,7 +236,8 @@ struct pxd_engine {
     // without tuning.  The tables survive pxd_reset, so repeated
     // decompositions of the same input reuse them.
     static constexpr int WV_LANES2 = 0, WV_WIN1 = 6, WV_WIN2 = 7, WV_WIN4 = 8, WV_WIN2S = 9, WV_WIN2C = 10,
-                         WV_WIN2R = 11, WV_WIN2RH = 12, WV_WIN2H = 13, WV_COUNT = 14;
+                         WV_WIN2R = 11, WV_WIN2RH = 12, WV_WIN2H = 13, WV_WIN1R = 14, WV_WIN1RH = 15,
+                         WV_COUNT = 16;
     static constexpr uint32_t RANGE_R = 8192;        // node-range chunks of variant 11 (12: half)
     uint32_t* d_rmap = nullptr;                      // k_range_offsets of the current list, RANGE_R
     uint32_t* d_rmap_h = nullptr;                    // the same for RANGE_R / 2
@@ -805,6 +808,14 @@ struct pxd_engine {
                 else launch_walk_win<2, 6, 10240, RANGE_R / 2, false, true>();
                 break;
             case WV_WIN2H: launch_walk_win<2, 6, 10240, 2048>(); break;
+            case WV_WIN1R:
+                if (serving) launch_walk_win<1, 8, 20480, 4096>();   // list not in index order
+                else launch_walk_win<1, 8, 20480, RANGE_R, false, true>();
+                break;
+            case WV_WIN1RH:
+                if (serving) launch_walk_win<1, 8, 20480, 4096>();
+                else launch_walk_win<1, 8, 10240, RANGE_R / 2, false, true>();
+                break;
             default: launch_walk_win<4, 8, 16384, 4096>(); break;
         }
     }
@@ -848,6 +859,11 @@ struct pxd_engine {
         }
         // half windows (two workgroups a CU) over 2048-entry chunks
         if (gs <= 8) cand.push_back(WV_WIN2H);
+        // one lane per row over node ranges: short rows, one round of 64 rows a wave
+        if (gs <= 4 && !serving) {
+            cand.push_back(WV_WIN1R);
+            cand.push_back(WV_WIN1RH);
+        }
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
@@ -903,6 +919,14 @@ struct pxd_engine {
                 else launch_contrib_win<2, 8, 20480, RANGE_R / 2, true>();
                 break;
             case WV_WIN2H: launch_contrib_win<2, 8, 20480, 2048>(); break;
+            case WV_WIN1R:
+                if (serving) launch_contrib_win<1, 8, 40960, 4096>();
+                else launch_contrib_win<1, 8, 40960, RANGE_R, true>();
+                break;
+            case WV_WIN1RH:
+                if (serving) launch_contrib_win<1, 8, 40960, 4096>();
+                else launch_contrib_win<1, 8, 20480, RANGE_R / 2, true>();
+                break;
             default: launch_contrib_win<4, 8, 32768, 4096>(); break;
         }
     }

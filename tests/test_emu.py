@@ -33,8 +33,10 @@ def test_emulated_engine_with_address_sanitizer_matches_oracle():
     lib = build_emu.build(asan=True)
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0")
     # small single graphs run in one workgroup (k_decompose_small); "/phases"
-    # forces the per-phase kernels on the same graph
-    cases = ["path100", "path100/phases", "sbm300", "sbm2500", "sbm2500/phases", "ingest_loops_dups",
+    # forces the per-phase kernels on the same graph (the 2500-node SBM only so:
+    # its one-workgroup run adds minutes under AddressSanitizer and is covered
+    # by the smaller graphs here and by the GPU tests)
+    cases = ["path100", "path100/phases", "sbm300", "sbm2500/phases", "ingest_loops_dups",
              "ingest_hub3000"]
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_emu_parity.py"), lib, *cases],
                        capture_output=True, text=True, timeout=800, env=env)

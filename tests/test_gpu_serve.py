@@ -184,12 +184,12 @@ def test_serve_edge_lists_equal_solo_and_csr_admissions():
     srv.close()
 
 
-@pytest.mark.parametrize("variant", [1, 7, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("variant", [1, 7, 9, 10, 11, 12, 13, 14, 15])
 def test_serve_pinned_kernel_variants_equal_solo(variant):
     """A serving engine's node list is a concatenation of the admitted graphs'
     lists, not one list in index order: every pinned kernel variant still
-    gives each graph its solo result (variants 11 and 12, which need the
-    list in index order, run as variant 7 there)."""
+    gives each graph its solo result (variants 11, 12, 14 and 15, which
+    need the list in index order, run as variants 7 and 6 there)."""
     graphs = mix()
     ref = [solo(g) for g in graphs]
     srv = px.Server(node_capacity=30000, arc_capacity=400000, graph_slots=16,
