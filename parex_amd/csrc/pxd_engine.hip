@@ -859,11 +859,9 @@ struct pxd_engine {
         }
         // half windows (two workgroups a CU) over 2048-entry chunks
         if (gs <= 8) cand.push_back(WV_WIN2H);
-        // one lane per row over node ranges: short rows, one round of 64 rows a wave
-        if (gs <= 4 && !serving) {
-            cand.push_back(WV_WIN1R);
-            cand.push_back(WV_WIN1RH);
-        }
+        // (14 and 15, one lane per row over node ranges, are pinnable only:
+        // pinned on the 512-SBM batch they were the slowest window kernels at
+        // every list length)
         int best = def;
         float best_ms = 0.f;
         for (int v : cand) {
