@@ -970,9 +970,13 @@ __global__ void __launch_bounds__(BS) k_scan_lb(uint32_t n, const uint32_t* __re
 }
 
 // Single-pass ranked selection, the status protocol of k_scan_lb: element i
-// of [0, n) has a flag f.flag(i), and every element gets f.emit(i, r, flag)
-// with r = the number of flagged elements before it; the flagged count goes
-// to *total (when given).  One launch replaces a flag kernel, a scan and a
+// of [0, n) has the values d = f.load(i) and the flag f.flag(i, d), and every
+// element gets f.emit(i, r, flag, d) with r = the number of flagged elements
+// before it; the flagged count goes to *total (when given).  load() does all
+// of an element's reads and emit() only writes: the kernel loads every row of
+// its tile before the first flag and emits from registers, so the rows' loads
+// are in flight together (rows past the end load the last element and are
+// dropped).  One launch replaces a flag kernel, a scan and a
 // scatter kernel (the flags and prefixes never go to memory).  A tile is
 // ROWS rows of BS consecutive elements, row q read by all threads (element
 // base + q * BS + t), so the predicate's loads coalesce; an element's rank
@@ -980,7 +984,7 @@ __global__ void __launch_bounds__(BS) k_scan_lb(uint32_t n, const uint32_t* __re
 // the wave ballot's count of flagged lanes below it.  Long lists take tiles
 // of SE_ROWS rows (few tiles for the look-back to cross), shorter ones
 // SE_ROWS_SHORT rows (enough workgroups to occupy the chip).
-constexpr uint32_t SE_ROWS = 32, SE_ROWS_SHORT = 8;
+constexpr uint32_t SE_ROWS = 16, SE_ROWS_SHORT = 8;
 constexpr uint32_t SE_LONG = 1u << 23;             // lists of at least this many elements: SE_ROWS
 template <uint32_t SE_ROWS, class F>
 __global__ void __launch_bounds__(BS) k_scan_emit(uint32_t n, F f, unsigned long long* status, uint32_t* ticket,
@@ -1000,12 +1004,19 @@ __global__ void __launch_bounds__(BS) k_scan_emit(uint32_t n, F f, unsigned long
     const uint32_t base = tile * SE_TILE;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
     const uint64_t lt = (1ull << l) - 1ull;
-    // all rows' flags first (their loads in flight together), then the ballots
+    // all rows' values and flags first (no branch per row: rows past the end
+    // load the last element), then the ballots
+    typename F::D d[SE_ROWS];
     uint32_t fl = 0u;
+    if (n != 0u) {
 #pragma unroll
-    for (uint32_t q = 0; q < SE_ROWS; ++q) {
-        const uint32_t i = base + q * BS + threadIdx.x;
-        if (i < n && f.flag(i)) fl |= 1u << q;
+        for (uint32_t q = 0; q < SE_ROWS; ++q) d[q] = f.load(min(base + q * BS + threadIdx.x, n - 1u));
+#pragma unroll
+        for (uint32_t q = 0; q < SE_ROWS; ++q) {
+            const uint32_t i = base + q * BS + threadIdx.x;
+            const bool fv = f.flag(min(i, n - 1u), d[q]);
+            if (i < n && fv) fl |= 1u << q;
+        }
     }
     uint32_t rk[SE_ROWS];
 #pragma unroll
@@ -1048,7 +1059,7 @@ __global__ void __launch_bounds__(BS) k_scan_emit(uint32_t n, F f, unsigned long
 #pragma unroll
     for (uint32_t q = 0; q < SE_ROWS; ++q) {
         const uint32_t i = base + q * BS + threadIdx.x;
-        if (i < n) f.emit(i, prefix + s_cnt[q * NW + w] + rk[q], ((fl >> q) & 1u) != 0u);
+        if (i < n) f.emit(i, prefix + s_cnt[q * NW + w] + rk[q], ((fl >> q) & 1u) != 0u, d[q]);
     }
     if (tile == ntiles - 1u && threadIdx.x == 0 && total) *total = prefix + tot;
 }
@@ -1062,12 +1073,18 @@ struct SelMembers {
     uint32_t iter;
     uint32_t* out;
     uint32_t* root;
-    __device__ bool flag(uint32_t i) const { return stamp[alist[i]] == iter; }
-    __device__ void emit(uint32_t i, uint32_t r, bool f) const {
+    struct D {
+        uint32_t u, s;
+    };
+    __device__ D load(uint32_t i) const {
+        const uint32_t u = alist[i];
+        return D{u, stamp[u]};
+    }
+    __device__ bool flag(uint32_t, const D& d) const { return d.s == iter; }
+    __device__ void emit(uint32_t, uint32_t r, bool f, const D& d) const {
         if (f) {
-            const uint32_t u = alist[i];
-            out[r] = u;
-            root[u] = u;
+            out[r] = d.u;
+            root[d.u] = d.u;
         }
     }
 };
@@ -1078,8 +1095,17 @@ struct SelOffGiant {
     const uint32_t* giant;
     const uint32_t* root;
     uint32_t* out;
-    __device__ bool flag(uint32_t i) const;
-    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = list[i]; }
+    struct D {
+        uint32_t u, g, p;             // member, its part's giant root, its pointer
+    };
+    __device__ D load(uint32_t i) const {
+        const uint32_t u = list[i];
+        return D{u, giant[part[u]], root[u]};
+    }
+    __device__ bool flag(uint32_t i, const D& d) const;
+    __device__ void emit(uint32_t, uint32_t r, bool f, const D& d) const {
+        if (f) out[r] = d.u;
+    }
 };
 // the positions of the kept clusters, their new cluster ids,
 // and every kept cluster's new first position (cstart, by old cluster id)
@@ -1090,13 +1116,19 @@ struct SelPositions {
     const uint32_t* kscan;
     const uint32_t* cs;
     uint32_t *pnix_o, *pclu_o, *cstart;
-    __device__ bool flag(uint32_t k) const { return ckeep[pclu[k]] != 0u; }
-    __device__ void emit(uint32_t k, uint32_t r, bool f) const {
-        if (!f) return;
+    struct D {
+        uint32_t c, u, keep, nc, first;   // cluster, node, kept, new cluster id, cluster start
+    };
+    __device__ D load(uint32_t k) const {
         const uint32_t c = pclu[k];
-        pnix_o[r] = pnix[k];
-        pclu_o[r] = kscan[c];
-        if (cs[c] == k) cstart[c] = r;
+        return D{c, pnix[k], ckeep[c], kscan[c], cs[c]};
+    }
+    __device__ bool flag(uint32_t, const D& d) const { return d.keep != 0u; }
+    __device__ void emit(uint32_t k, uint32_t r, bool f, const D& d) const {
+        if (!f) return;
+        pnix_o[r] = d.u;
+        pclu_o[r] = d.nc;
+        if (d.first == k) cstart[d.c] = r;
     }
 };
 // the active nodes still alive
@@ -1104,8 +1136,17 @@ struct SelAlive {
     const uint32_t* alist;
     const uint8_t* alive;
     uint32_t* out;
-    __device__ bool flag(uint32_t i) const { return alive[alist[i]] != 0; }
-    __device__ void emit(uint32_t i, uint32_t r, bool f) const { if (f) out[r] = alist[i]; }
+    struct D {
+        uint32_t u, a;
+    };
+    __device__ D load(uint32_t i) const {
+        const uint32_t u = alist[i];
+        return D{u, alive[u]};
+    }
+    __device__ bool flag(uint32_t, const D& d) const { return d.a != 0u; }
+    __device__ void emit(uint32_t, uint32_t r, bool f, const D& d) const {
+        if (f) out[r] = d.u;
+    }
 };
 // the cluster table from runs of equal regroup keys (sorted by minimum node):
 // a run starts a cluster, every position gets its run's id
@@ -1114,15 +1155,18 @@ struct SelRuns {
     const uint32_t* key;
     const uint32_t* ngraph;
     uint32_t *cs, *cmin, *cgraph, *cflag, *pclu;
-    __device__ bool flag(uint32_t k) const { return k == 0 || key[k] != key[k - 1]; }
-    __device__ void emit(uint32_t k, uint32_t r, bool f) const {
+    struct D {
+        uint32_t key, prev;
+    };
+    __device__ D load(uint32_t k) const { return D{key[k], key[k ? k - 1u : 0u]}; }
+    __device__ bool flag(uint32_t k, const D& d) const { return (k == 0u) | (d.key != d.prev); }
+    __device__ void emit(uint32_t k, uint32_t r, bool f, const D& d) const {
         const uint32_t c = r + (f ? 1u : 0u) - 1u;
         pclu[k] = c;
         if (f) {
-            const uint32_t kk = key[k];
             cs[c] = k;
-            cmin[c] = kk;
-            cgraph[c] = ngraph[kk];
+            cmin[c] = d.key;
+            cgraph[c] = ngraph[d.key];
             cflag[c] = CF_CONN;
         }
         if (k == na - 1u) cs[c + 1u] = na;
@@ -1988,9 +2032,8 @@ __global__ void k_cc_giant(uint32_t nc, const uint32_t* __restrict__ cs, const u
 }
 
 // a member outside its part's giant tree
-__device__ __forceinline__ bool SelOffGiant::flag(uint32_t i) const {
-    const uint32_t u = list[i];
-    return cc_find_stable(root, u) != giant[part[u]];
+__device__ __forceinline__ bool SelOffGiant::flag(uint32_t, const D& d) const {
+    return (d.p == d.u ? d.u : cc_find_stable(root, d.p)) != d.g;
 }
 
 // Exact rounds over the members outside the giant trees, GS lanes per node:

@@ -81,7 +81,7 @@ def build(asan=False):
     for old, new in (("constexpr uint32_t SMALL_SCAN_N = 1024 * SMALL_SCAN_ITEMS;", "constexpr uint32_t SMALL_SCAN_N = 0;"),
                      ("constexpr uint32_t LB_SCAN_ITEMS = 32;", "constexpr uint32_t LB_SCAN_ITEMS = 4;"),
                      ("constexpr uint32_t SORT_SMALL_USE = SORT_SMALL_N;", "constexpr uint32_t SORT_SMALL_USE = 1024;"),
-                     ("constexpr uint32_t SE_ROWS = 32, SE_ROWS_SHORT = 8;",
+                     ("constexpr uint32_t SE_ROWS = 16, SE_ROWS_SHORT = 8;",
                       "constexpr uint32_t SE_ROWS = 4, SE_ROWS_SHORT = 2;"),
                      ("constexpr uint32_t SE_LONG = 1u << 23;", "constexpr uint32_t SE_LONG = 1500;")):
         assert kh.count(old) == 1, old
