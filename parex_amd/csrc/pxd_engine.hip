@@ -1106,7 +1106,8 @@ struct pxd_engine {
         // then the stable regroup): min and max are the first and last keys
         k_stats_prep<<<gc, BS, 0, st>>>(ncb, T->cs, T->cgraph, d_pnix[0], d_xkey, d_gfirst, d_cmaxo, d_cmino, d_cK,
                                         d_csum, pnc);
-        k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum);
+        k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum,
+                                                                 d_nclu);
         k_stats_final<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold, d_ckeep, d_cavg,
                                          pnc);
         HIPC(hipGetLastError());
@@ -1115,8 +1116,8 @@ struct pxd_engine {
                                     T->clabel, d_gseen, pnc);
         k_graph_update<<<grid_for(G, BS), BS, 0, st>>>(G, d_gseen, d_gsmall, d_gsmall_next, d_gsurv, d_giter,
                                                        d_gstart, iter1);
-        // the active node list holds the same nodes as the positions: apply in index order
-        k_node_cluster<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_nclu);
+        // the active node list holds the same nodes as the positions: apply in
+        // index order (k_stats_sum recorded every node's cluster)
         k_apply_nodes<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_nclu, d_ckeep, T->clabel, d_cavg,
                                                            d_dint, d_label, d_x[cur], d_y[cur], d_alive, &d_cnt->err);
         HIPC(hipGetLastError());

@@ -2215,11 +2215,14 @@ __global__ void k_stats_prep(uint32_t nc, const uint32_t* __restrict__ cs, const
 // order of the additions does not matter.  Four positions a thread: twice
 // the workgroups of eight and half the atomics of two (C4: k_stats_sum 91 us
 // and k_volume 73 us a launch, against 99 and 81 with eight).
+// k_stats_sum also records every position's cluster by node (nclu, for the
+// node-order apply pass), while it has the position's node and cluster.
 constexpr uint32_t SUM_ITEMS = 4;
 __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* __restrict__ pnix,
                                                   const uint32_t* __restrict__ pclu,
                                                   const double* __restrict__ x, const int32_t* __restrict__ cK,
-                                                  unsigned long long* __restrict__ csum) {
+                                                  unsigned long long* __restrict__ csum,
+                                                  uint32_t* __restrict__ nclu) {
     __shared__ int64_t lds[BS / WAVE];
     constexpr uint32_t TR = BS * SUM_ITEMS;
     for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
@@ -2231,7 +2234,11 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
 #pragma unroll
             for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
                 const uint32_t k = r0 + i * BS + threadIdx.x;
-                if (k < r1) acc += (int64_t)ldexp((double)(float)x[pnix[k]], K);
+                if (k < r1) {
+                    const uint32_t u = pnix[k];
+                    acc += (int64_t)ldexp((double)(float)x[u], K);
+                    nclu[u] = cf;
+                }
             }
             acc = group_sum<WAVE>(acc);
             if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
@@ -2249,7 +2256,11 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
             const bool ok = k < r1;
             const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
             int64_t term = 0;
-            if (ok) term = (int64_t)ldexp((double)(float)x[pnix[k]], cK[c]);
+            if (ok) {
+                const uint32_t u = pnix[k];
+                term = (int64_t)ldexp((double)(float)x[u], cK[c]);
+                nclu[u] = c;
+            }
             const uint32_t c0 = __shfl(c, 0);
             if (c0 == 0xFFFFFFFFu) {
                 continue;                      // whole wave past the end: nothing to add
@@ -2337,12 +2348,6 @@ __global__ void k_graph_update(uint32_t G, const uint32_t* __restrict__ gactive,
 }
 
 // per position: the cluster of its node (nclu[pnix[k]] = pclu[k])
-__global__ void k_node_cluster(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
-                               uint32_t* __restrict__ nclu) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x)
-        nclu[pnix[k]] = pclu[k];
-}
-
 // per active node, in index order: write labels, recentre survivors
 // (x -= avg), refresh y; deactivated nodes lose `alive`
 __global__ void k_apply_nodes(uint32_t n, const uint32_t* __restrict__ alist, const uint32_t* __restrict__ nclu,

@@ -504,7 +504,8 @@ __global__ void __launch_bounds__(SM_NT) k_decompose_small(SmallArgs args) {
         gsmall = s_lab32;
         if (nsurv == 0u && giter == 0) giter = (int32_t)iter1 + 1;
         const uint32_t nc_new = nsurv;
-        // cluster of every node, labels, recentring, deactivation (k_node_cluster, k_apply_nodes)
+        // cluster of every node, labels, recentring, deactivation (the node
+        // clusters k_stats_sum records, k_apply_nodes)
         for (uint32_t k = t; k < na; k += SM_NT) a.nclu[pnix[k]] = pclu[k];
         __syncthreads();
         for (uint32_t e = t; e < nalist; e += SM_NT) {
