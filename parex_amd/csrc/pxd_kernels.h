@@ -1629,18 +1629,31 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
     __syncthreads();
     const uint32_t lastc = pclu[b1 - 1];
     uint32_t ed = 0u, vol = 0u, st = 0u;
+    // every item's cluster and node, then their cluster starts and node values
+    // (loads in flight together; items past the end read the last position)
+    uint32_t c[SCAN_ITEMS], u[SCAN_ITEMS], cst[SCAN_ITEMS];
+    uint2 ev[SCAN_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint32_t k = min(b0 + threadIdx.x * SCAN_ITEMS + i, b1 - 1u);
+        c[i] = pclu[k];
+        u[i] = pnix[k];
+    }
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        cst[i] = cs[c[i]];
+        ev[i] = edv[u[i]];            // the cut contribution and dInt, one gather
+    }
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         const uint32_t k = b0 + threadIdx.x * SCAN_ITEMS + i;
         if (k >= b1) break;
-        const uint32_t c = pclu[k];
-        if (cs[c] == k) st = 1u;
+        if (cst[i] == k) st = 1u;
         // the node's values gathered once, kept in position order for k_sweep_c
-        const uint32_t u = pnix[k];
-        const uint2 ev = edv[u];      // the cut contribution and dInt, one gather
-        const uint32_t e = ev.x, d = ev.y;
+        const uint32_t e = ev[i].x, d = ev[i].y;
         ped[k] = e;
         pvol[k] = d;
-        if (c == lastc) {
+        if (c[i] == lastc) {
             ed += e;
             vol += d;
         }
@@ -2228,6 +2241,18 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
     for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
         const uint32_t r1 = min(r0 + TR, na);
         const uint32_t cf = pclu[r0];
+        // every item's node and cluster, then every item's value (loads in
+        // flight together; items past the end read the last position)
+        uint32_t u[SUM_ITEMS], ci[SUM_ITEMS];
+        double xv[SUM_ITEMS];
+#pragma unroll
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
+            const uint32_t k = min(r0 + i * BS + threadIdx.x, r1 - 1u);
+            u[i] = pnix[k];
+            ci[i] = pclu[k];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) xv[i] = x[u[i]];
         if (cf == pclu[r1 - 1]) {
             const int32_t K = cK[cf];
             int64_t acc = 0;
@@ -2235,9 +2260,8 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
             for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
                 const uint32_t k = r0 + i * BS + threadIdx.x;
                 if (k < r1) {
-                    const uint32_t u = pnix[k];
-                    acc += (int64_t)ldexp((double)(float)x[u], K);
-                    nclu[u] = cf;
+                    acc += (int64_t)ldexp((double)(float)xv[i], K);
+                    nclu[u[i]] = cf;
                 }
             }
             acc = group_sum<WAVE>(acc);
@@ -2254,12 +2278,11 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
         for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
             const uint32_t k = r0 + i * BS + threadIdx.x;
             const bool ok = k < r1;
-            const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
+            const uint32_t c = ok ? ci[i] : 0xFFFFFFFFu;
             int64_t term = 0;
             if (ok) {
-                const uint32_t u = pnix[k];
-                term = (int64_t)ldexp((double)(float)x[u], cK[c]);
-                nclu[u] = c;
+                term = (int64_t)ldexp((double)(float)xv[i], cK[c]);
+                nclu[u[i]] = c;
             }
             const uint32_t c0 = __shfl(c, 0);
             if (c0 == 0xFFFFFFFFu) {
@@ -2615,13 +2638,17 @@ __global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __re
     for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
         const uint32_t r1 = min(r0 + TR, na);
         const uint32_t cf = pclu[r0];
+        // every item's node, then every item's degree (loads in flight
+        // together; items past the end read the last position and add 0)
+        uint32_t u[SUM_ITEMS], dv[SUM_ITEMS];
+#pragma unroll
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) u[i] = pnix[min(r0 + i * BS + threadIdx.x, r1 - 1u)];
+#pragma unroll
+        for (uint32_t i = 0; i < SUM_ITEMS; ++i) dv[i] = r0 + i * BS + threadIdx.x < r1 ? dint[u[i]] : 0u;
         if (cf == pclu[r1 - 1]) {
             uint32_t acc = 0;
 #pragma unroll
-            for (uint32_t i = 0; i < SUM_ITEMS; ++i) {
-                const uint32_t k = r0 + i * BS + threadIdx.x;
-                if (k < r1) acc += dint[pnix[k]];
-            }
+            for (uint32_t i = 0; i < SUM_ITEMS; ++i) acc += dv[i];
             acc = group_sum<WAVE>(acc);
             if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
             __syncthreads();
@@ -2637,7 +2664,7 @@ __global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __re
             const uint32_t k = r0 + i * BS + threadIdx.x;
             const bool ok = k < r1;
             const uint32_t c = ok ? pclu[k] : 0xFFFFFFFFu;
-            uint32_t d = ok ? dint[pnix[k]] : 0u;
+            uint32_t d = dv[i];
             const uint32_t c0 = __shfl(c, 0);
             if (c0 == 0xFFFFFFFFu) {
                 continue;                      // whole wave past the end: nothing to add
