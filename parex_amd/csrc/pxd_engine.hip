@@ -1002,7 +1002,7 @@ struct pxd_engine {
                 // valid arcs of active nodes: sum of dInt over the node list
                 HIPC(hipMemsetAsync(d_u64, 0, 8, st));
                 // (dInt of inactive nodes is stale; sum only over the active list)
-                k_gather<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, d_tC);
+                k_gather<<<grid_for(nalist, BS * GATHER_ITEMS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, d_tC);
                 k_sum_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tC, d_u64);
                 unsigned long long arcs = 0;
                 HIPC(hipMemcpyAsync(&arcs, d_u64, 8, hipMemcpyDeviceToHost, st));
@@ -1029,7 +1029,7 @@ struct pxd_engine {
         ncc_members = 0;
         if (na == 0 || nc == 0) { sweep_done = true; return 0; }
         // keys in position order, then stable sort inside every cluster
-        k_gather<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
+        k_gather<<<grid_for(na, BS * GATHER_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
         HIPC(hipGetLastError());
         CHK(seg_sort(T.cs, nc, na));
         CHK(heavy_map());

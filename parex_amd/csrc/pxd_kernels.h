@@ -912,10 +912,17 @@ __global__ void __launch_bounds__(BS) k_scan_lb(uint32_t n, const uint32_t* __re
     const uint32_t ntiles = n ? (n - 1u) / LB_SCAN_TILE + 1u : 1u;   // tile 0 writes out[0] when n == 0
     if (tile >= ntiles) return;
     const uint32_t base = tile * LB_SCAN_TILE;
+    {
+        // every element into registers first (elements past the end read the
+        // last one and count 0), so the loads are in flight together
+        uint32_t t[LB_SCAN_ITEMS];
 #pragma unroll
-    for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
-        const uint32_t k = i * BS + threadIdx.x;
-        sm[k + (k >> 5)] = base + k < n ? in[base + k] : 0u;
+        for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) t[i] = n ? in[min(base + i * BS + threadIdx.x, n - 1u)] : 0u;
+#pragma unroll
+        for (uint32_t i = 0; i < LB_SCAN_ITEMS; ++i) {
+            const uint32_t k = i * BS + threadIdx.x;
+            sm[k + (k >> 5)] = base + k < n ? t[i] : 0u;
+        }
     }
     __syncthreads();
     const uint32_t k0 = threadIdx.x * LB_SCAN_ITEMS + threadIdx.x * LB_SCAN_ITEMS / 32u;
@@ -1298,13 +1305,23 @@ __global__ void __launch_bounds__(BS) k_sort_block(const uint32_t* __restrict__ 
     __syncthreads();
     uint64_t e[TILE_ITEMS];
     uint32_t lor = 0u, land = 0xFFFFFFFFu;
+    {
+        // every item loaded first (past the end: the last item, dropped), so
+        // the loads are in flight together
+        uint32_t kv[TILE_ITEMS], vv[TILE_ITEMS];
 #pragma unroll
-    for (int i = 0; i < TILE_ITEMS; ++i) {
-        const uint32_t p = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l;
-        const bool ok = p < cnt;
-        const uint32_t k = ok ? key[s0 + p] : 0u;
-        e[i] = ((uint64_t)k << 32) | (ok ? val[s0 + p] : 0u);
-        if (ok) { lor |= k; land &= k; }
+        for (int i = 0; i < TILE_ITEMS; ++i) {
+            const uint32_t p = min(w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l, cnt - 1u);
+            kv[i] = key[s0 + p];
+            vv[i] = val[s0 + p];
+        }
+#pragma unroll
+        for (int i = 0; i < TILE_ITEMS; ++i) {
+            const bool ok = w * (WAVE * TILE_ITEMS) + (uint32_t)i * WAVE + l < cnt;
+            const uint32_t k = ok ? kv[i] : 0u;
+            e[i] = ((uint64_t)k << 32) | (ok ? vv[i] : 0u);
+            if (ok) { lor |= k; land &= k; }
+        }
     }
     atomicOr(&s_or, lor);
     atomicAnd(&s_and, land);
@@ -1552,10 +1569,14 @@ __global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ p
     // a wave whose lanes share one digit adds its count with one atomic: the
     // regroup's keys come in runs (a cluster's positions share its minimum),
     // and 64 lanes on one LDS counter would serialise
-    for (uint32_t p0 = b; p0 < e; p0 += BS) {
-        const uint32_t p = p0 + threadIdx.x;
+    uint32_t kv[TILE / BS];           // the tile's keys first, their loads in flight together
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)(TILE / BS); ++q) kv[q] = key[min(b + q * BS + threadIdx.x, e - 1u)];
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)(TILE / BS); ++q) {
+        const uint32_t p = b + q * BS + threadIdx.x;
         const bool ok = p < e;
-        const uint32_t d = ok ? (key[p] >> shift) & 255u : 256u;
+        const uint32_t d = ok ? (kv[q] >> shift) & 255u : 256u;
         const uint32_t d0 = __shfl(d, 0);
         const uint64_t same = __ballot(ok && d == d0);
         if (same == __ballot(ok)) {
@@ -1729,16 +1750,25 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
     uint32_t ed[SCAN_ITEMS], vo[SCAN_ITEMS], cc[SCAN_ITEMS], uu[SCAN_ITEMS];
     bool st[SCAN_ITEMS], ok[SCAN_ITEMS];
     uint32_t lf = 0u, le = 0u, lv = 0u;
+    // every item's values, then their clusters' starts, before any use (loads
+    // in flight together; items past the end read the last position, na >= 1)
+    uint32_t cst[SCAN_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint32_t k = min(b0 + t * SCAN_ITEMS + i, na - 1u);
+        cc[i] = pclu[k];
+        uu[i] = nd ? pnix[k] : 0u;
+        ed[i] = ped[k];
+        vo[i] = pvol[k];
+    }
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) cst[i] = cs[cc[i]];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         const uint32_t k = b0 + t * SCAN_ITEMS + i;
         ok[i] = k < na;
         if (ok[i]) {
-            cc[i] = pclu[k];
-            uu[i] = nd ? pnix[k] : 0u;
-            st[i] = cs[cc[i]] == k;
-            ed[i] = ped[k];
-            vo[i] = pvol[k];
+            st[i] = cst[i] == k;
         } else {
             cc[i] = 0u; uu[i] = 0u; st[i] = true; ed[i] = 0u; vo[i] = 0u;
         }
@@ -2678,10 +2708,22 @@ __global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __re
     }
 }
 
-__global__ void k_gather(uint32_t n, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ src,
-                         uint32_t* __restrict__ dst) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        dst[i] = src[idx[i]];
+// dst[i] = src[idx[i]]: GATHER_ITEMS elements a thread, all index loads, then
+// all gathers in flight together (launched with BS threads a workgroup)
+constexpr uint32_t GATHER_ITEMS = 4;
+__global__ void __launch_bounds__(BS) k_gather(uint32_t n, const uint32_t* __restrict__ idx,
+                                               const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+    constexpr uint32_t TR = BS * GATHER_ITEMS;
+    for (uint32_t i0 = blockIdx.x * TR + threadIdx.x; i0 < n; i0 += gridDim.x * TR) {
+        uint32_t j[GATHER_ITEMS], v[GATHER_ITEMS];
+#pragma unroll
+        for (uint32_t q = 0; q < GATHER_ITEMS; ++q) j[q] = idx[min(i0 + q * BS, n - 1u)];
+#pragma unroll
+        for (uint32_t q = 0; q < GATHER_ITEMS; ++q) v[q] = src[j[q]];
+#pragma unroll
+        for (uint32_t q = 0; q < GATHER_ITEMS; ++q)
+            if (i0 + q * BS < n) dst[i0 + q * BS] = v[q];
+    }
 }
 
 __global__ void k_fill_u32(uint32_t n, uint32_t* __restrict__ p, uint32_t v, const uint32_t* pn = nullptr) {
