@@ -2008,9 +2008,34 @@ __global__ void __launch_bounds__(WBS) k_cc_sample_win(uint32_t nl, const uint32
                 ok[k] = i0 + k * WBS < c1;
                 u[k] = list[ok[k] ? i0 + k * WBS : c0];
             }
+            if (r == 0u) {
+                // every member's row bounds, then all their sampled arcs, then the
+                // kept ones out (loads in flight together, as cc_sample_arc does
+                // for one member)
+                uint32_t dd[SI], rw[SI], a[SI][CC_SAMPLE_ROUNDS];
+#pragma unroll
+                for (int k = 0; k < SI; ++k) {
+                    dd[k] = ok[k] ? dint[u[k]] : 0u;
+                    rw[k] = row[u[k]];
+                }
+#pragma unroll
+                for (int k = 0; k < SI; ++k)
+#pragma unroll
+                    for (uint32_t j = 0; j < CC_SAMPLE_ROUNDS; ++j) a[k][j] = dd[k] > j ? nbr[rw[k] + j] : INVALID;
+#pragma unroll
+                for (int k = 0; k < SI; ++k) {
+                    if (ok[k]) {
+#pragma unroll
+                        for (uint32_t j = 1; j < CC_SAMPLE_ROUNDS; ++j) samp[(j - 1u) * nl + i0 + k * WBS] = a[k][j];
+                    }
+                    v[k] = ok[k] ? a[k][0] : INVALID;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < SI; ++k) v[k] = ok[k] ? samp[(r - 1u) * nl + i0 + k * WBS] : INVALID;
+            }
 #pragma unroll
             for (int k = 0; k < SI; ++k) {
-                v[k] = ok[k] ? cc_sample_arc(r, i0 + k * WBS, u[k], nl, row, dint, nbr, samp) : INVALID;
                 ok[k] = v[k] != INVALID;
                 if (!ok[k]) v[k] = u[k];
             }
