@@ -182,13 +182,14 @@ int pxd_set_small_graph_kernel(pxd_engine* e, int enable);
  * lanes, 2048-entry chunks and half the window (two workgroups a CU); 14,
  * 15: variants 11, 12 with one lane per row (a serving engine runs variant 6
  * for them); -1 (default): online
- * tuning -- the first time the active list has a length in [2^b, 2^(b+1))
- * (b >= 16) every candidate is launched three times on the current state and timed,
- * the fastest is kept for that pass and length class (the tables survive
+ * tuning -- the first time the active list has a length in a quarter of
+ * [2^b, 2^(b+1)) (b >= 16; quarters by the two bits after the leading one)
+ * every candidate is launched three times on the current state and timed,
+ * the fastest is kept for that pass and length (the tables survive
  * pxd_reset); -2: the degree rule without tuning.  Every variant computes the
  * same bits.  pxd_walk_tuning copies the tables (entries 0..32: walk per
- * class b, 33..65: cut contribution; -1 = not tuned) and the number of
- * tunings run. */
+ * class b, 33..65: cut contribution, each the choice for the class's
+ * highest tuned quarter; -1 = not tuned) and the number of tunings run. */
 int pxd_set_walk_variant(pxd_engine* e, int variant);
 int pxd_walk_tuning(pxd_engine* e, int32_t* table, uint32_t capacity, uint64_t* tunings);
 

@@ -246,13 +246,22 @@ struct pxd_engine {
     static constexpr uint32_t TUNE_MIN = 1u << 16;   // shorter lists: launch cost dominates, no tuning
     int walk_force = -1;
     int small_kernel = 1;              // one-workgroup runs: 0 never, 1 up to SMALL_AUTO_ARCS, 2 always
-    int tuned[2][33];
+    // tuned[pass][b][q]: the list lengths in [2^b, 2^(b+1)) in four quarters by
+    // the two bits after the leading one (a long class such as 2^24 spans a
+    // dozen iterations whose best kernel changes)
+    static constexpr int TUNE_Q = 4;
+    int tuned[2][33][TUNE_Q];
+    int& tuned_slot(int pass, uint32_t n) {
+        const int b = 31 - __builtin_clz(n);
+        const int q = b >= 2 ? (int)((n >> (b - 2)) & 3u) : 0;
+        return tuned[pass][b][q];
+    }
     uint64_t tuned_for[3] = {~0ull, ~0ull, ~0ull};   // (graphs, nodes, arcs) the tables were tuned on
     uint64_t walk_tunings = 0;
     double avg_deg = 0.0;
     hipEvent_t tev0 = nullptr, tev1 = nullptr;
 
-    pxd_engine() { std::fill(&tuned[0][0], &tuned[0][0] + 66, -1); }
+    pxd_engine() { std::fill(&tuned[0][0][0], &tuned[0][0][0] + 2 * 33 * TUNE_Q, -1); }
     ~pxd_engine() {
         release();
         trim_pool();
@@ -471,7 +480,7 @@ struct pxd_engine {
         // the online tuner's tables belong to the input's sizes: a new input of
         // the same sizes (the same workload uploaded again) keeps them
         if (tuned_for[0] != G || tuned_for[1] != nn || tuned_for[2] != aa) {
-            std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+            std::fill(&tuned[0][0][0], &tuned[0][0][0] + 2 * 33 * TUNE_Q, -1);
             tuned_for[0] = G;
             tuned_for[1] = nn;
             tuned_for[2] = aa;
@@ -528,7 +537,7 @@ struct pxd_engine {
         release();
         serving = false;
         HIPC(hipSetDevice(dev));
-        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+        std::fill(&tuned[0][0][0], &tuned[0][0][0] + 2 * 33 * TUNE_Q, -1);
         tuned_for[0] = tuned_for[1] = tuned_for[2] = ~0ull;
         if (n >= 0x7FFFFFFFu || 2 * m >= 0x7FFFFFFFull)
             return set_err(PXD_ERR_ARG, "device CSR build takes below 2^31 nodes and 2^31 arcs (%u nodes, %llu edges)",
@@ -839,8 +848,8 @@ struct pxd_engine {
         if (walk_force >= 0) return walk_force;
         const int def = default_variant();
         if (walk_force == -2 || nalist < TUNE_MIN) return def;
-        const int b = 31 - __builtin_clz(nalist);
-        if (tuned[pass][b] >= 0) return tuned[pass][b];
+        int& slot = tuned_slot(pass, nalist);
+        if (slot >= 0) return slot;
         std::vector<int> cand;
         const int lv = lanes_variant(gs);
         for (int v = std::max(0, lv - 1); v <= std::min(5, lv + 1); ++v) cand.push_back(v);
@@ -879,7 +888,7 @@ struct pxd_engine {
                 }
             }
         }
-        tuned[pass][b] = best;
+        slot = best;
         ++walk_tunings;
         return best;
     }
@@ -1186,7 +1195,7 @@ struct pxd_engine {
         // kernel (neighbours near in index order) and the list fills the chip;
         // at any length when a window variant is pinned
         const int wv = walk_force >= 0 ? walk_force
-                                       : (nalist >= TUNE_MIN ? tuned[0][31 - __builtin_clz(nalist)] : -1);
+                                       : (nalist >= TUNE_MIN ? tuned_slot(0, nalist) : -1);
         const bool win = wv >= WV_WIN1 && (walk_force >= 0 || nl >= (1u << 20));
         for (uint32_t r = 0; r < CC_SAMPLE_ROUNDS; ++r) {
             if (win) {
@@ -1426,7 +1435,7 @@ struct pxd_engine {
         heavy_thr = 128u;
         heavy_valid = false;
         rmap_valid = false;
-        std::fill(&tuned[0][0], &tuned[0][0] + 66, -1);
+        std::fill(&tuned[0][0][0], &tuned[0][0][0] + 2 * 33 * TUNE_Q, -1);
         tuned_for[0] = tuned_for[1] = tuned_for[2] = ~0ull;
         serving = true;
         inited = true;
@@ -2161,7 +2170,13 @@ int pxd_set_walk_variant(pxd_engine* e, int variant) {
 int pxd_walk_tuning(pxd_engine* e, int32_t* table, uint32_t capacity, uint64_t* tunings) {
     return guarded([&]() -> int {
         if (!e) return set_err(PXD_ERR_ARG, "null engine");
-        for (uint32_t b = 0; b < capacity && b < 66; ++b) table[b] = e->tuned[b / 33][b % 33];
+        // per class, the choice for its longest lists tuned so far (the
+        // highest quarter that has one)
+        for (uint32_t b = 0; b < capacity && b < 66; ++b) {
+            int v = -1;
+            for (int q = pxd_engine::TUNE_Q - 1; q >= 0 && v < 0; --q) v = e->tuned[b / 33][b % 33][q];
+            table[b] = v;
+        }
         if (tunings) *tunings = e->walk_tunings;
         return 0;
     });
