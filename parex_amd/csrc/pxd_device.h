@@ -248,6 +248,25 @@ __device__ __forceinline__ XcdSpan xcd_span(uint32_t n, uint32_t per, uint32_t l
     return XcdSpan{b + lb * per + local, e, nb * per};
 }
 
+// The tiles of a tile loop over n items (tiles of `tile` consecutive items),
+// split the same way: XCD x strides over the x-th eighth of the tiles.  Loops
+// over positions use it: a cluster is a contiguous position range, so its
+// gathers and scattered writes to node arrays (the cluster's nodes) stay in
+// one XCD's L2 instead of touching the same lines from all eight.  The grid
+// must be a multiple of 8.
+__device__ __forceinline__ XcdSpan xcd_tiles(uint32_t n, uint32_t tile) {
+    return xcd_span(n / tile + (n % tile != 0), 1, 0);
+}
+
+// blockIdx.x renumbered so that the blocks of each XCD (b % 8 == x) take one
+// contiguous range of ids; a bijection of 0 .. gridDim.x - 1 for any grid.
+// For kernels that give each block one tile by its id.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb / NXCD, r = nb % NXCD, x = b % NXCD;
+    return x * q + min(x, r) + b / NXCD;
+}
+
 // Device-coherent (agent scope) relaxed accesses for words that other threads
 // of the same launch may write.  Each XCD has its own L2; plain accesses to
 // such words are data races, these compile to sc1 accesses that every XCD

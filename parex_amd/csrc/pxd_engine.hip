@@ -1011,7 +1011,7 @@ struct pxd_engine {
                 // valid arcs of active nodes: sum of dInt over the node list
                 HIPC(hipMemsetAsync(d_u64, 0, 8, st));
                 // (dInt of inactive nodes is stale; sum only over the active list)
-                k_gather<<<grid_for(nalist, BS * GATHER_ITEMS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, d_tC);
+                k_gather<<<grid_xcd(nalist, BS * GATHER_ITEMS), BS, 0, st>>>(nalist, d_alist[acur], d_dint, d_tC);
                 k_sum_u32<<<grid_for(nalist, BS), BS, 0, st>>>(nalist, d_tC, d_u64);
                 unsigned long long arcs = 0;
                 HIPC(hipMemcpyAsync(&arcs, d_u64, 8, hipMemcpyDeviceToHost, st));
@@ -1038,7 +1038,7 @@ struct pxd_engine {
         ncc_members = 0;
         if (na == 0 || nc == 0) { sweep_done = true; return 0; }
         // keys in position order, then stable sort inside every cluster
-        k_gather<<<grid_for(na, BS * GATHER_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
+        k_gather<<<grid_xcd(na, BS * GATHER_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_xkey, d_pkey[0]);
         HIPC(hipGetLastError());
         CHK(seg_sort(T.cs, nc, na));
         CHK(heavy_map());
@@ -1090,7 +1090,7 @@ struct pxd_engine {
         const uint32_t* pnc = nullptr;        // the device count when ncb is a bound
         if (ncc > 0) {
             nl = ncc_members;
-            k_mark<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
+            k_mark<<<grid_xcd(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cs, T->cflag, d_coff, iter1, d_part,
                                                     d_root, d_stamp);
             // the CC list in node order (the active list holds the positions' nodes);
             // its length is the member count the sweep returned
@@ -1099,7 +1099,7 @@ struct pxd_engine {
             CHK(connected_components(nl, *T));
             // regroup: stable sort of all positions by component / cluster minimum
             k_cc_flatten<<<grid_for(nl, BS), BS, 0, st>>>(nl, d_cclist, d_root);
-            k_rid<<<grid_for(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
+            k_rid<<<grid_xcd(na, BS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], T->cflag, T->cmin, d_root, d_pkey[0]);
             k_one_segment<<<1, 64, 0, st>>>(d_oneseg, na, nullptr);
             HIPC(hipGetLastError());
             CHK(seg_sort(d_oneseg, 1, na));
@@ -1115,7 +1115,7 @@ struct pxd_engine {
         // then the stable regroup): min and max are the first and last keys
         k_stats_prep<<<gc, BS, 0, st>>>(ncb, T->cs, T->cgraph, d_pnix[0], d_xkey, d_gfirst, d_cmaxo, d_cmino, d_cK,
                                         d_csum, pnc);
-        k_stats_sum<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum,
+        k_stats_sum<<<grid_xcd(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_x[cur], d_cK, d_csum,
                                                                  d_nclu);
         k_stats_final<<<gc, BS, 0, st>>>(ncb, T->cs, d_cmaxo, d_cmino, d_cK, d_csum, P.rw_threshold, d_ckeep, d_cavg,
                                          pnc);
@@ -1174,7 +1174,7 @@ struct pxd_engine {
         }
         if (nc) {                     // k_clu_compact cleared the volumes
             ClusterTable& T3 = ct[ctc];
-            k_volume<<<grid_for(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
+            k_volume<<<grid_xcd(na, BS * SUM_ITEMS), BS, 0, st>>>(na, d_pnix[0], d_pclu[0], d_dint, T3.cvol);
             HIPC(hipGetLastError());
         }
         return 0;

@@ -1644,7 +1644,8 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
                                                 uint32_t* __restrict__ tail_ed, uint32_t* __restrict__ tail_vol,
                                                 uint32_t* __restrict__ has_start) {
     __shared__ uint32_t s_ed, s_vol, s_start;
-    const uint32_t b0 = blockIdx.x * (BS * SCAN_ITEMS);
+    const uint32_t tb = xcd_block();          // this block's tile
+    const uint32_t b0 = tb * (BS * SCAN_ITEMS);
     const uint32_t b1 = min(na, b0 + BS * SCAN_ITEMS);
     if (threadIdx.x == 0) { s_ed = 0u; s_vol = 0u; s_start = 0u; }
     __syncthreads();
@@ -1689,9 +1690,9 @@ __global__ void __launch_bounds__(BS) k_sweep_a(uint32_t na, const uint32_t* __r
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        tail_ed[blockIdx.x] = s_ed;
-        tail_vol[blockIdx.x] = s_vol;
-        has_start[blockIdx.x] = s_start;
+        tail_ed[tb] = s_ed;
+        tail_vol[tb] = s_vol;
+        has_start[tb] = s_start;
     }
 }
 
@@ -1745,7 +1746,8 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
                                                 unsigned long long* __restrict__ cbest,
                                                 pxd_node_data* __restrict__ nd) {
     __shared__ uint32_t f[BS], e[BS], v[BS];
-    const uint32_t b0 = blockIdx.x * (BS * SCAN_ITEMS);
+    const uint32_t tb = xcd_block();          // this block's tile
+    const uint32_t b0 = tb * (BS * SCAN_ITEMS);
     const uint32_t t = threadIdx.x;
     uint32_t ed[SCAN_ITEMS], vo[SCAN_ITEMS], cc[SCAN_ITEMS], uu[SCAN_ITEMS];
     bool st[SCAN_ITEMS], ok[SCAN_ITEMS];
@@ -1790,7 +1792,7 @@ __global__ void __launch_bounds__(BS) k_sweep_c(uint32_t na, const uint32_t* __r
     bool open;                         // prefix still continues a segment from before the block
     if (t == 0) { re = 0u; rv = 0u; open = true; }
     else { re = e[t - 1]; rv = v[t - 1]; open = !f[t - 1]; }
-    if (open) { re += carry_ed[blockIdx.x]; rv += carry_vol[blockIdx.x]; }
+    if (open) { re += carry_ed[tb]; rv += carry_vol[tb]; }
     uint32_t runc = 0xFFFFFFFFu;
     unsigned long long runmin = ~0ull;
 #pragma unroll
@@ -1896,7 +1898,8 @@ __global__ void __launch_bounds__(BS) k_mark(uint32_t na, const uint32_t* __rest
                                              const uint32_t* __restrict__ coff, uint32_t iter,
                                              uint32_t* __restrict__ part, uint32_t* __restrict__ root,
                                              uint32_t* __restrict__ stamp) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+    const XcdSpan sp = xcd_span(na, BS, threadIdx.x);
+    for (uint32_t k = sp.first; k < sp.end; k += sp.stride) {
         const uint32_t c = pclu[k];
         const uint32_t f = cflag[c];
         if (f & CF_CC) {
@@ -2227,7 +2230,8 @@ __global__ void k_cc_flatten(uint32_t nl, const uint32_t* __restrict__ list, uin
 __global__ void k_rid(uint32_t na, const uint32_t* __restrict__ pnix, const uint32_t* __restrict__ pclu,
                       const uint32_t* __restrict__ cflag, const uint32_t* __restrict__ cmin,
                       const uint32_t* __restrict__ root, uint32_t* __restrict__ key) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+    const XcdSpan sp = xcd_span(na, BS, threadIdx.x);
+    for (uint32_t k = sp.first; k < sp.end; k += sp.stride) {
         const uint32_t c = pclu[k];
         key[k] = (cflag[c] & CF_CC) ? root[pnix[k]] : cmin[c];
     }
@@ -2293,7 +2297,9 @@ __global__ void __launch_bounds__(BS) k_stats_sum(uint32_t na, const uint32_t* _
                                                   uint32_t* __restrict__ nclu) {
     __shared__ int64_t lds[BS / WAVE];
     constexpr uint32_t TR = BS * SUM_ITEMS;
-    for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
+    const XcdSpan sp = xcd_tiles(na, TR);
+    for (uint32_t t = sp.first; t < sp.end; t += sp.stride) {
+        const uint32_t r0 = t * TR;
         const uint32_t r1 = min(r0 + TR, na);
         const uint32_t cf = pclu[r0];
         // every item's node and cluster, then every item's value (loads in
@@ -2690,7 +2696,9 @@ __global__ void __launch_bounds__(BS) k_volume(uint32_t na, const uint32_t* __re
     __shared__ uint32_t lds[BS / WAVE];
     na = dev_count(na, pn);
     constexpr uint32_t TR = BS * SUM_ITEMS;
-    for (uint32_t r0 = blockIdx.x * TR; r0 < na; r0 += gridDim.x * TR) {
+    const XcdSpan sp = xcd_tiles(na, TR);
+    for (uint32_t t = sp.first; t < sp.end; t += sp.stride) {
+        const uint32_t r0 = t * TR;
         const uint32_t r1 = min(r0 + TR, na);
         const uint32_t cf = pclu[r0];
         // every item's node, then every item's degree (loads in flight
@@ -2739,7 +2747,9 @@ constexpr uint32_t GATHER_ITEMS = 4;
 __global__ void __launch_bounds__(BS) k_gather(uint32_t n, const uint32_t* __restrict__ idx,
                                                const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
     constexpr uint32_t TR = BS * GATHER_ITEMS;
-    for (uint32_t i0 = blockIdx.x * TR + threadIdx.x; i0 < n; i0 += gridDim.x * TR) {
+    const XcdSpan sp = xcd_tiles(n, TR);
+    for (uint32_t t = sp.first; t < sp.end; t += sp.stride) {
+        const uint32_t i0 = t * TR + threadIdx.x;
         uint32_t j[GATHER_ITEMS], v[GATHER_ITEMS];
 #pragma unroll
         for (uint32_t q = 0; q < GATHER_ITEMS; ++q) j[q] = idx[min(i0 + q * BS, n - 1u)];
