@@ -749,10 +749,10 @@ struct pxd_engine {
             int src = 0;
             for (int pass = 0; pass < 4; ++pass) {   // four passes: the result is back in buffer 0
                 const int sh = 8 * pass;
-                k_tile_hist<<<tiles, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_pkey[src], sh, d_hist);
+                k_tile_hist<<<(tiles + NXCD - 1) / NXCD * NXCD, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_pkey[src], sh, d_hist);
                 HIPC(hipGetLastError());
                 CHK(scan(d_hist, d_hscan, tiles * 256u, &d_cnt->tiles256));
-                k_tile_scatter<<<tiles, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_hscan, d_pkey[src],
+                k_tile_scatter<<<(tiles + NXCD - 1) / NXCD * NXCD, BS, 0, st>>>(&d_cnt->tiles, d_tseg, d_tidx, d_ls, d_hscan, d_pkey[src],
                                                      d_pnix[src], sh, d_pkey[src ^ 1], d_pnix[src ^ 1]);
                 HIPC(hipGetLastError());
                 src ^= 1;

@@ -1559,11 +1559,14 @@ __global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ p
                                                   const uint32_t* __restrict__ key, int shift,
                                                   uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
-    if (blockIdx.x >= *pntiles) return;   // uniform over the workgroup
+    // XCD x takes the x-th eighth of the tiles (a segment's tiles are
+    // consecutive, so its lines stay in one L2); the grid is a multiple of 8
+    const XcdSpan sp = xcd_span(*pntiles, 1, 0);
+  for (uint32_t ti = sp.first; ti < sp.end; ti += sp.stride) {
     h[threadIdx.x] = 0u;
     __syncthreads();
-    const LargeSeg L = ls[tseg[blockIdx.x]];
-    const uint32_t t = tidx[blockIdx.x];
+    const LargeSeg L = ls[tseg[ti]];
+    const uint32_t t = tidx[ti];
     const uint32_t b = L.start + t * TILE;
     const uint32_t e = min(L.start + L.len, b + (uint32_t)TILE);
     // a wave whose lanes share one digit adds its count with one atomic: the
@@ -1587,6 +1590,7 @@ __global__ void __launch_bounds__(BS) k_tile_hist(const uint32_t* __restrict__ p
     }
     __syncthreads();
     hist[L.hbase + threadIdx.x * L.ntiles + t] = h[threadIdx.x];
+  }
 }
 
 // One LSD pass over one tile (rank8), then the scatter to the digit's global
@@ -1603,9 +1607,11 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(const uint32_t* __restrict_
     __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t hs[256], goff[256];
     __shared__ uint32_t sred[NW];
-    if (blockIdx.x >= *pntiles) return;   // uniform over the workgroup
-    const LargeSeg L = ls[tseg[blockIdx.x]];
-    const uint32_t t = tidx[blockIdx.x];
+    // tiles split per XCD as in k_tile_hist; the grid is a multiple of 8
+    const XcdSpan sp = xcd_span(*pntiles, 1, 0);
+  for (uint32_t ti = sp.first; ti < sp.end; ti += sp.stride) {
+    const LargeSeg L = ls[tseg[ti]];
+    const uint32_t t = tidx[ti];
     const uint32_t b = L.start + t * TILE;
     const uint32_t cnt = min(L.start + L.len, b + (uint32_t)TILE) - b;
     const uint32_t w = threadIdx.x / WAVE, l = lane_id();
@@ -1630,6 +1636,8 @@ __global__ void __launch_bounds__(BS) k_tile_scatter(const uint32_t* __restrict_
         kdst[dst] = (uint32_t)(x >> 32);
         vdst[dst] = (uint32_t)x;
     }
+    __syncthreads();                  // buf, goff and hs are rewritten for the next tile
+  }
 }
 
 // ------------------------------------------------------------------ sweep scan
