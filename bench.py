@@ -4,9 +4,10 @@ A step is one complete decomposition of the workload's graph set (every graph
 run to its final partition), timed as the reference's CLI times it
 (main.cpp:42-48): upload of the host CSR (pinned host buffers) + initial
 state + decomposition.  `value` is the rate of a stream of `--steps` such
-batches on two engines, batch k+1's upload overlapping batch k's
-decomposition (`--no-stream`: one engine, upload then decomposition, which
-is `value_serial`); `value_resident` is the decomposition alone with the
+batches on `--engines` engines (4), each with its own HIP stream and host
+thread, so that one batch's upload and short late-iteration kernels overlap
+the others' work (`--no-stream`: one engine, upload then decomposition,
+which is `value_serial`); `value_resident` is the decomposition alone with the
 input already in HBM.
 
 Default workload (`--workload sbm512`): 512 independent planted-partition
@@ -413,52 +414,52 @@ def batched_leg(pinned, warmup, steps):
             "ids": ids, "ks": ks, "its": its, "walk": walk, "walk_log": walk_log, "tuning": tuning}
 
 
-def streamed_leg(pinned, warmup, steps, batch):
-    """A stream of `steps` batches on two engines, each on its own HIP stream:
-    while one engine decomposes batch k, a host thread uploads batch k+1 into
-    the other (pinned host buffers, initial state), so the host-link copies
-    overlap the decomposition.  The timed region runs from the first batch's
-    upload to the last batch's final partition; every batch is the whole
-    graph set.  Also checks both engines' partitions equal `batch`'s."""
+def streamed_leg(pinned, warmup, steps, batch, engines=4):
+    """A stream of `steps` batches over `engines` engines, each engine on its
+    own HIP stream and host thread, batch k on engine k % engines: each engine
+    uploads its batch (pinned host buffers, initial state) and decomposes it,
+    while the other engines' uploads and decompositions run beside it, so the
+    host-link copies and the short late-iteration kernels of one batch overlap
+    another's work.  The timed region runs from the first upload to the last
+    batch's final partition; every batch is the whole graph set.  Also checks
+    every engine's partition equals `batch`'s."""
     import threading
-    dms = [px.DeviceManager(px.Params(keep_original=False, record_node_data=False)) for _ in range(2)]
+    E = max(1, min(engines, steps))
+    dms = [px.DeviceManager(px.Params(keep_original=False, record_node_data=False)) for _ in range(E)]
     for dm in dms:                    # each engine runs its own tuning launches here
         for _ in range(max(warmup, 1)):
             dm.initialize_batch(pinned)
             dm.expanderDecomposition()
-    err = []
+    err, its = [], [0] * steps
 
-    def upload(dm):
+    def run(e):
         try:
-            dm.initialize_batch(pinned)
-        except Exception as e:        # reported on the main thread
-            err.append(e)
+            for k in range(e, steps, E):
+                dms[e].initialize_batch(pinned)
+                its[k] = dms[e].expanderDecomposition()
+        except Exception as ex:       # reported on the main thread
+            err.append(ex)
 
     t0 = time.perf_counter()
-    dms[0].initialize_batch(pinned)
-    its = []
-    for k in range(steps):
-        th = None
-        if k + 1 < steps:
-            th = threading.Thread(target=upload, args=(dms[(k + 1) % 2],))
-            th.start()
-        its.append(dms[k % 2].expanderDecomposition())
-        if th is not None:
-            th.join()
-        if err:
-            raise err[0]
+    ths = [threading.Thread(target=run, args=(e,)) for e in range(E)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
     t1 = time.perf_counter()
+    if err:
+        raise err[0]
     same = True
-    for dm in dms[:min(steps, 2)]:
+    for dm in dms:
         ids, ks = dm.getFinalPartition()
         same = same and np.array_equal(ids, batch["ids"]) and np.array_equal(np.atleast_1d(ks), batch["ks"]) \
             and np.array_equal(dm.graphIterations(), batch["its"])
     for dm in dms:
         dm.close()
     ms = (t1 - t0) * 1e3 / steps
-    log(f"streamed: {steps} batches in {(t1 - t0) * 1e3:.1f} ms = {ms:.1f} ms a batch, iterations {max(its)}; "
-        f"equal to the single-engine steps: {same}")
-    return {"step_ms": ms, "same": same}
+    log(f"streamed: {steps} batches on {E} engines in {(t1 - t0) * 1e3:.1f} ms = {ms:.1f} ms a batch, "
+        f"iterations {max(its)}; equal to the single-engine steps: {same}")
+    return {"step_ms": ms, "same": same, "engines": E}
 
 
 def plain_leg(pinned, batch):
@@ -485,12 +486,13 @@ def plain_leg(pinned, batch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sbm512",
                     choices=["sbm512", "torus1024", "rmat22", "sbm4096", "serve", "hierarchy"])
     ap.add_argument("--graphs", type=int, default=None, help="use only the first G graphs of the set")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--engines", type=int, default=4, help="batch workloads: engines of the streamed leg")
     ap.add_argument("--no-stream", action="store_true",
                     help="batch workloads: value from the single-engine steps (upload, then decomposition)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -524,16 +526,17 @@ def main():
 
     b = batched_leg(pinned, a.warmup, a.steps)
     serial_value = m_tot / (b["step_ms"] / 1e3)
-    s = None if a.no_stream else streamed_leg(pinned, a.warmup, a.steps, b)
+    s = None if a.no_stream else streamed_leg(pinned, a.warmup, a.steps, b, a.engines)
     if s is not None and not s["same"]:
         log("streamed batches differ from the single-engine steps")
         return 1
     step_ms = s["step_ms"] if s is not None else b["step_ms"]
     value = m_tot / (step_ms / 1e3)
     if s is not None:
-        cfg["timed"] = ("a stream of `steps` batches: upload from pinned host memory + initial state + "
-                        "decomposition of every batch, batch k+1's upload overlapping batch k's decomposition "
-                        "(two engines, two HIP streams); from the first upload to the last partition")
+        cfg["timed"] = (f"a stream of `steps` batches: upload from pinned host memory + initial state + "
+                        f"decomposition of every batch, on {s['engines']} engines with a HIP stream and a host "
+                        f"thread each, their batches running side by side; from the first upload to the last "
+                        f"partition")
     launches, walk_ms, walk_bytes = b["walk"]
     tuning, ctuning, tunings = b["tuning"]
     achieved = walk_bytes / (walk_ms / 1e3) / 1e9 if walk_ms > 0 else 0.0

@@ -24,7 +24,7 @@ def _asan_runtime():
     return path if os.path.isabs(path) and os.path.exists(path) else None
 
 
-@pytest.mark.timeout(900)
+@pytest.mark.timeout(1800)
 def test_emulated_engine_with_address_sanitizer_matches_oracle():
     asan = _asan_runtime()
     if asan is None:
@@ -32,13 +32,17 @@ def test_emulated_engine_with_address_sanitizer_matches_oracle():
     import build_emu
     lib = build_emu.build(asan=True)
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0")
-    # small single graphs run in one workgroup (k_decompose_small); "/phases"
-    # forces the per-phase kernels on the same graph (the 2500-node SBM only so:
-    # its one-workgroup run adds minutes under AddressSanitizer and is covered
-    # by the smaller graphs here and by the GPU tests)
-    cases = ["path100", "path100/phases", "sbm300", "sbm2500/phases", "ingest_loops_dups",
-             "ingest_hub3000"]
+    # "/phases" forces the per-phase kernels (the 100-node path: waves wholly
+    # past the end of the position list, the regression of 4b7b85d); the edge
+    # lists run the device CSR build.  Each emulated decomposition takes about
+    # two minutes under AddressSanitizer, so the default suite runs these; the
+    # one-workgroup runs (path100, sbm300) and the large-segment sort plan
+    # (sbm2500/phases) run with PXD_EMU_CASES=all (the GPU tests cover them too).
+    cases = ["path100/phases", "ingest_loops_dups", "ingest_hub3000"]
+    if os.environ.get("PXD_EMU_CASES") == "all":
+        cases = ["path100", "path100/phases", "sbm300", "sbm2500/phases", "ingest_loops_dups",
+                 "ingest_hub3000"]
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_emu_parity.py"), lib, *cases],
-                       capture_output=True, text=True, timeout=800, env=env)
+                       capture_output=True, text=True, timeout=1700, env=env)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-6000:]
     assert r.stdout.count("equal") == len(cases) and "ERROR: AddressSanitizer" not in r.stderr
