@@ -65,6 +65,7 @@ struct pxh_hierarchy {
     // device buffers of the first (largest) level serve all later ones; freed
     // by pxh_destroy
     pxd_engine* engine = nullptr;
+    pxd_params engine_params{};         // the parameters `engine` was created with
 
     ~pxh_hierarchy() {
         if (engine) pxd_destroy(engine);
@@ -345,14 +346,19 @@ static int build_levels(pxh_hierarchy* h, const pxd_params* params, float target
     pxd_params p;
     if (params) p = *params;
     else pxd_default_params(&p);
-    if (h->engine) {
+    // a later build with the same parameters keeps the engine (its device
+    // buffers and kernel tuning): every level re-initializes it anyway
+    if (h->engine && std::memcmp(&h->engine_params, &p, sizeof p) != 0) {
         pxd_destroy(h->engine);
         h->engine = nullptr;
     }
-    if (int rc = pxd_create(&p, &h->engine)) {
-        h->engine = nullptr;
-        g_err = std::string("pxd_create: ") + pxd_last_error();
-        return rc;
+    if (!h->engine) {
+        if (int rc = pxd_create(&p, &h->engine)) {
+            h->engine = nullptr;
+            g_err = std::string("pxd_create: ") + pxd_last_error();
+            return rc;
+        }
+        h->engine_params = p;
     }
     float sparsity = target_sparsity;
     uint32_t retries = 0;
